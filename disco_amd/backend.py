@@ -1,0 +1,384 @@
+"""Python mirror of the reference's backend interface, on the HIP C ABI.
+
+Mirrors ``Code/backend.hpp``:
+
+* ``LoopData``       -- Code/backend.hpp:20-26 (loopons SoA or polyline + apex)
+* ``BackendTiming``  -- Code/backend.hpp:36-39
+* ``GpuOptions``     -- Code/backend.hpp:53-56 (``fp32`` selects DISCO_PREC_FP32C;
+                        ``chunk_pairs`` is accepted and ignored: one launch covers
+                        all loops, and no watchdog needs chunking here)
+* ``gpu_available``, ``print_gpu_info``, ``gpu_line_fp64``, ``kroupa_gpu``
+                     -- Code/backend.hpp:57-63 / Code/backend_cuda.cu:525-722
+
+``kroupa_gpu`` keeps the reference's contract: the caller owns the host arrays,
+``ux, uy, uz`` end holding (their input +) the sum over loops and images.  All
+compute goes through ``libdisco_hip.so``; there is no CPU path here -- a
+missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdisco_hip.so")
+
+PREC_FP64 = 0
+PREC_FP32C = 1
+
+ERR_NAMES = {0: "DISCO_OK", 1: "DISCO_ERR_ARG", 2: "DISCO_ERR_NO_DEVICE", 3: "DISCO_ERR_HIP",
+             4: "DISCO_ERR_OOM", 5: "DISCO_ERR_TOO_LARGE", 6: "DISCO_ERR_STATE"}
+
+
+# ------------------------------------------------------------------ types
+@dataclass
+class LoopData:
+    """One loop for the backend (Code/backend.hpp:20-26)."""
+    phys: tuple = (0.0,) * 10            # bx by bz mx my mz nx ny nz dS
+    line: bool = False
+    x: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    y: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    z: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    vx: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    vy: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    vz: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    cx: float = 0.0
+    cy: float = 0.0
+    cz: float = 0.0
+
+    def burgers(self):
+        """b = (bx mx, by my, bz mz), Code/backend_cpu.cpp:44-45."""
+        p = self.phys
+        return (p[0] * p[3], p[1] * p[4], p[2] * p[5])
+
+
+@dataclass
+class BackendTiming:
+    upload: float = 0.0
+    kernels: float = 0.0
+    download: float = 0.0
+    kernel_ms: float = 0.0
+    loop_sec: list = field(default_factory=list)
+
+
+@dataclass
+class GpuOptions:
+    fp32: bool = False
+    chunk_pairs: float = -1.0
+
+
+class DiscoError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+# ------------------------------------------------------------------ C ABI
+class _LineLoop(ctypes.Structure):
+    _fields_ = [("nv", ctypes.c_int), ("vx", ctypes.POINTER(ctypes.c_double)),
+                ("vy", ctypes.POINTER(ctypes.c_double)), ("vz", ctypes.POINTER(ctypes.c_double)),
+                ("c", ctypes.c_double * 3), ("b", ctypes.c_double * 3)]
+
+
+class _LooponLoop(ctypes.Structure):
+    _fields_ = [("np", ctypes.c_int), ("x", ctypes.POINTER(ctypes.c_double)),
+                ("y", ctypes.POINTER(ctypes.c_double)), ("z", ctypes.POINTER(ctypes.c_double)),
+                ("phys", ctypes.c_double * 10)]
+
+
+class _Timing(ctypes.Structure):
+    _fields_ = [("upload", ctypes.c_double), ("kernels", ctypes.c_double),
+                ("download", ctypes.c_double), ("kernel_ms", ctypes.c_double),
+                ("loop_sec", ctypes.POINTER(ctypes.c_double))]
+
+
+class _DeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 256), ("compute_units", ctypes.c_int),
+                ("global_mem", ctypes.c_size_t), ("lds_per_block", ctypes.c_size_t),
+                ("arch_major", ctypes.c_int), ("arch_minor", ctypes.c_int),
+                ("arch", ctypes.c_char * 64)]
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+
+
+def lib():
+    """libdisco_hip.so; raises if it was not built (there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP backend library missing: {LIB_PATH} "
+                               "(build it with __graft_entry__.build())")
+        # One HIP runtime per process: torch links its bundled libamdhip64 by
+        # the unversioned name, this library by SONAME libamdhip64.so.7.  Loading
+        # torch first makes the SONAME resolve to torch's copy, so device
+        # pointers and stream handles from torch are valid here.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = ctypes.CDLL(LIB_PATH)
+        L.disco_hip_device_count.argtypes = [ctypes.POINTER(_i)]
+        L.disco_hip_device_info.argtypes = [_i, ctypes.POINTER(_DeviceInfo)]
+        L.disco_hip_line_fp64.argtypes = []
+        L.disco_hip_strerror.argtypes = [_i]
+        L.disco_hip_strerror.restype = ctypes.c_char_p
+        L.disco_hip_open.argtypes = [_i, ctypes.POINTER(_vp)]
+        L.disco_hip_close.argtypes = [_vp]
+        L.disco_hip_last_error.argtypes = [_vp]
+        L.disco_hip_last_error.restype = ctypes.c_char_p
+        L.disco_hip_line_sum.argtypes = [_vp, _i, _dp, _dp, _dp, _dp, _dp, _dp,
+                                         ctypes.POINTER(_LineLoop), _i, _d, _d, _d, _i, _i,
+                                         ctypes.POINTER(_Timing)]
+        L.disco_hip_loopon_sum.argtypes = [_vp, _i, _dp, _dp, _dp, _dp, _dp, _dp,
+                                           ctypes.POINTER(_LooponLoop), _i, _d, _d, _d, _i, _i,
+                                           ctypes.POINTER(_Timing)]
+        L.disco_hip_plan_line.argtypes = [_vp, ctypes.POINTER(_LineLoop), _i, _d, _d, _d, _i, _i,
+                                          ctypes.POINTER(_vp)]
+        L.disco_hip_plan_loopon.argtypes = [_vp, ctypes.POINTER(_LooponLoop), _i, _d, _d, _d, _i,
+                                            _i, ctypes.POINTER(_vp)]
+        L.disco_hip_plan_run.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]
+        L.disco_hip_plan_profile.argtypes = [_vp, _i]
+        L.disco_hip_plan_kernel_ms.argtypes = [_vp, _dp, _i]
+        L.disco_hip_plan_evals_per_atom.argtypes = [_vp]
+        L.disco_hip_plan_evals_per_atom.restype = _d
+        L.disco_hip_plan_free.argtypes = [_vp]
+        _lib = L
+    return _lib
+
+
+def _check(code, ctx=None):
+    if code != 0:
+        L = lib()
+        detail = L.disco_hip_last_error(ctx).decode() if ctx else ""
+        raise DiscoError(code, (L.disco_hip_strerror(code).decode() +
+                                (f" ({detail})" if detail else "")))
+
+
+# ------------------------------------------------------------------ devices
+def gpu_available() -> bool:
+    """A HIP device can be used (Code/backend_cuda.cu:537-541)."""
+    n = _i(0)
+    return lib().disco_hip_device_count(ctypes.byref(n)) == 0 and n.value > 0
+
+
+def gpu_info(device: int = 0) -> dict:
+    info = _DeviceInfo()
+    _check(lib().disco_hip_device_info(device, ctypes.byref(info)))
+    return {"name": info.name.decode(), "compute_units": info.compute_units,
+            "global_mem": info.global_mem, "lds_per_block": info.lds_per_block,
+            "arch": info.arch.decode()}
+
+
+def print_gpu_info(device: int = 0) -> None:
+    """The "--- GPU: <name> ---" banner (Code/backend_cuda.cu:543-553), in CUs."""
+    g = gpu_info(device)
+    print(f"\n--- GPU: {g['name']} ({g['arch']}) ---")
+    print(f"  CUs              : {g['compute_units']}")
+    print(f"  Global memory    : {g['global_mem'] >> 20} MB")
+    print(f"  LDS/block        : {g['lds_per_block'] // 1024} KB\n")
+
+
+def gpu_line_fp64() -> bool:
+    return bool(lib().disco_hip_line_fp64())
+
+
+# ------------------------------------------------------------------ context
+class Context:
+    """A device context of the C ABI (one stream, cached buffers)."""
+
+    def __init__(self, device: int = 0):
+        self._h = _vp()
+        _check(lib().disco_hip_open(device, ctypes.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().disco_hip_close(self._h)
+            self._h = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, code):
+        _check(code, self._h)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def line_structs(loops):
+    """ctypes array of disco_line_loop; keeps the numpy buffers alive."""
+    keep = []
+    arr = (_LineLoop * max(1, len(loops)))()
+    for i, lp in enumerate(loops):
+        vx, vy, vz = _f64(lp.vx), _f64(lp.vy), _f64(lp.vz)
+        keep += [vx, vy, vz]
+        arr[i].nv = int(vx.shape[0])
+        arr[i].vx = vx.ctypes.data_as(_dp)
+        arr[i].vy = vy.ctypes.data_as(_dp)
+        arr[i].vz = vz.ctypes.data_as(_dp)
+        arr[i].c[:] = [lp.cx, lp.cy, lp.cz]
+        arr[i].b[:] = list(lp.burgers())
+    return arr, keep
+
+
+def loopon_structs(loops):
+    keep = []
+    arr = (_LooponLoop * max(1, len(loops)))()
+    for i, lp in enumerate(loops):
+        x, y, z = _f64(lp.x), _f64(lp.y), _f64(lp.z)
+        keep += [x, y, z]
+        arr[i].np = int(x.shape[0])
+        arr[i].x = x.ctypes.data_as(_dp)
+        arr[i].y = y.ctypes.data_as(_dp)
+        arr[i].z = z.ctypes.data_as(_dp)
+        arr[i].phys[:] = [float(t) for t in lp.phys]
+    return arr, keep
+
+
+def _sum_host(ctx, kind, x, y, z, u, loops, box, img, prec, timing):
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    n = x.shape[0]
+    ux, uy, uz = u
+    for a in (ux, uy, uz):
+        if a.dtype != np.float64 or not a.flags.c_contiguous or a.shape[0] != n:
+            raise ValueError("u arrays must be C-contiguous float64 of length n")
+    t = _Timing()
+    sec = np.zeros(max(1, len(loops)))
+    t.loop_sec = sec.ctypes.data_as(_dp)
+    if kind == "line":
+        arr, keep = line_structs(loops)
+        fn = lib().disco_hip_line_sum
+    else:
+        arr, keep = loopon_structs(loops)
+        fn = lib().disco_hip_loopon_sum
+    ctx.check(fn(ctx.handle, n, x.ctypes.data_as(_dp), y.ctypes.data_as(_dp),
+                 z.ctypes.data_as(_dp), ux.ctypes.data_as(_dp), uy.ctypes.data_as(_dp),
+                 uz.ctypes.data_as(_dp), arr, len(loops), float(box[0]), float(box[1]),
+                 float(box[2]), int(img), int(prec), ctypes.byref(t)))
+    del keep
+    if timing is not None:
+        timing.upload, timing.kernels, timing.download = t.upload, t.kernels, t.download
+        timing.kernel_ms = t.kernel_ms
+        timing.loop_sec = list(sec[:len(loops)])
+    return ux, uy, uz
+
+
+def line_sum(ctx, x, y, z, loops, box, img=1, prec=PREC_FP64, u=None, timing=None):
+    """Host-buffer line sum: u (+)= sum over loops x images (one launch)."""
+    n = len(x)
+    u = (np.zeros(n), np.zeros(n), np.zeros(n)) if u is None else u
+    return _sum_host(ctx, "line", x, y, z, u, loops, box, img, prec, timing)
+
+
+def loopon_sum(ctx, x, y, z, loops, box, img=1, prec=PREC_FP64, u=None, timing=None):
+    """Host-buffer loopon sum: u (+)= sum over loops x images x loopons."""
+    n = len(x)
+    u = (np.zeros(n), np.zeros(n), np.zeros(n)) if u is None else u
+    return _sum_host(ctx, "loopon", x, y, z, u, loops, box, img, prec, timing)
+
+
+class Plan:
+    """Device-resident geometry of one sample (disco_hip_plan_*)."""
+
+    def __init__(self, ctx, loops, box, img=1, prec=PREC_FP32C, kind=None):
+        self.ctx = ctx
+        self._h = _vp()
+        kind = kind or ("line" if (loops and loops[0].line) else "loopon")
+        self.kind = kind
+        if kind == "line":
+            arr, keep = line_structs(loops)
+            ctx.check(lib().disco_hip_plan_line(ctx.handle, arr, len(loops), float(box[0]),
+                                                float(box[1]), float(box[2]), int(img),
+                                                int(prec), ctypes.byref(self._h)))
+        else:
+            arr, keep = loopon_structs(loops)
+            ctx.check(lib().disco_hip_plan_loopon(ctx.handle, arr, len(loops), float(box[0]),
+                                                  float(box[1]), float(box[2]), int(img),
+                                                  int(prec), ctypes.byref(self._h)))
+        del keep
+
+    def profile(self, on: bool = True):
+        """Record HIP events around each kernel of later runs."""
+        self.ctx.check(lib().disco_hip_plan_profile(self._h, 1 if on else 0))
+
+    def kernel_ms(self):
+        """(main pass ms, near pass ms) of the last profiled run (waits for it)."""
+        ms = (ctypes.c_double * 2)()
+        self.ctx.check(lib().disco_hip_plan_kernel_ms(self._h, ms, 2))
+        return float(ms[0]), float(ms[1])
+
+    def evals_per_atom(self) -> float:
+        return lib().disco_hip_plan_evals_per_atom(self._h)
+
+    def run_ptr(self, n, px, py, pz, pux, puy, puz, accumulate=False, stream=None):
+        """Launch on device pointers (ints), asynchronously on `stream` (int or None)."""
+        self.ctx.check(lib().disco_hip_plan_run(self._h, int(n), px, py, pz, pux, puy, puz,
+                                                1 if accumulate else 0, stream))
+
+    def run_torch(self, x, y, z, ux, uy, uz, accumulate=False, stream=None):
+        """Launch on torch CUDA float64 tensors, on torch's current stream by default."""
+        import torch
+        for t in (x, y, z, ux, uy, uz):
+            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+                raise ValueError("device arrays must be contiguous float64 CUDA tensors")
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        self.run_ptr(x.numel(), x.data_ptr(), y.data_ptr(), z.data_ptr(), ux.data_ptr(),
+                     uy.data_ptr(), uz.data_ptr(), accumulate, stream)
+
+    def free(self):
+        if self._h:
+            lib().disco_hip_plan_free(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def kroupa_gpu(x, y, z, ux, uy, uz, loops, dx, dy, dz, opt: GpuOptions | None = None,
+               timing: BackendTiming | None = None, device: int = 0, img: int = 1):
+    """kroupaGPU (Code/backend_cuda.cu:568-722): every loop, 27 images (img = 1),
+    line and loopon loops alike; ux, uy, uz accumulate the result."""
+    opt = opt or GpuOptions()
+    prec = PREC_FP32C if opt.fp32 else PREC_FP64
+    line = [l for l in loops if l.line]
+    lps = [l for l in loops if not l.line]
+    tl, tp = BackendTiming(), BackendTiming()
+    with Context(device) as ctx:
+        if line:
+            line_sum(ctx, x, y, z, line, (dx, dy, dz), img, prec, (ux, uy, uz), tl)
+        if lps:
+            loopon_sum(ctx, x, y, z, lps, (dx, dy, dz), img, prec, (ux, uy, uz), tp)
+    if timing is not None:
+        timing.upload = tl.upload + tp.upload
+        timing.kernels = tl.kernels + tp.kernels
+        timing.download = tl.download + tp.download
+        timing.kernel_ms = tl.kernel_ms + tp.kernel_ms
+        it_l, it_p = iter(tl.loop_sec), iter(tp.loop_sec)
+        timing.loop_sec = [next(it_l) if l.line else next(it_p) for l in loops]
+    return ux, uy, uz

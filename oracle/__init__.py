@@ -1,0 +1,300 @@
+"""CPU oracle for the displacement hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
+leg may import this package, and only as the checker.  The product
+(``disco_amd``) never imports it and has no CPU fallback.
+
+Two libraries are wrapped:
+
+* ``liboracle.so`` -- the repo's FP64 restatement (``disco_oracle.c``), built
+  here and shipped with the tree;
+* ``_ref/libdisco_ref.so`` -- the reference's own ``Code/backend_cpu.cpp`` and
+  ``Code/loop.cpp`` compiled where they lie (``oracle/Makefile``, target
+  ``ref``).  Present only where the reference tree was available at build time.
+
+Parity pins of each oracle function (checked in ``tests/test_oracle_*.py``):
+
+============================  =================================================
+oracle function               pinned by
+============================  =================================================
+barnett_loop (FP64)           golden vectors of barnettLoop<double> (reference
+                              build); Dunavant quadrature of the Kroupa kernel;
+                              jump -b across the cut; apex / start invariance;
+                              on-axis solid angle of a regular pyramid; disc
+                              limit; linearity in b; collinear subdivision
+kroupa_pair (FP64)            golden vectors of kroupaPair; oddness in rho;
+                              rho < 1e-10 -> 0; loopon sum -> line integral
+line_sum (+-img images)       golden kroupaCPU output at +-1 (4,096-atom C1
+                              subsample); +-N = sum of single-image fields
+loopon_sum (+-img images)     golden kroupaCPU output at +-1 (C3 loop, 4,096
+                              atoms); +-N = sum of single-image fields
+line_sum_ref32                golden barnettLoop<float> vectors (the float
+                              instantiation the reference's --fp32 uses)
+============================  =================================================
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(_HERE, "liboracle.so")
+REF_SO = os.path.join(_HERE, "_ref", "libdisco_ref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+_d = ctypes.c_double
+_i = ctypes.c_int
+_f = ctypes.c_float
+
+_lib = None
+_ref = None
+
+
+def _ptr(a, kind=_dp):
+    return a.ctypes.data_as(kind)
+
+
+def lib():
+    """The FP64 restatement (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run `make -C oracle`)")
+        L = ctypes.CDLL(ORACLE_SO)
+        L.oracle_threads.restype = _i
+        L.oracle_k1.restype = _d
+        L.oracle_barnett_loop.argtypes = [_d, _d, _d, _dp, _dp, _dp, _i, _d, _d, _d, _d, _d, _d, _dp]
+        L.oracle_barnett_loop_f.argtypes = [_f, _f, _f, _fp, _fp, _fp, _i, _f, _f, _f, _f, _f, _f, _fp]
+        L.oracle_kroupa_pair.argtypes = [_d, _d, _d, _dp, _d, _d, _dp]
+        L.oracle_line_sum.argtypes = [_i, _dp, _dp, _dp, _dp, _dp, _dp, _i, _ip, _dp, _dp, _dp,
+                                      _dp, _dp, _d, _d, _d, _i]
+        L.oracle_loopon_sum.argtypes = [_i, _dp, _dp, _dp, _dp, _dp, _dp, _i, _ip, _dp, _dp, _dp,
+                                        _dp, _d, _d, _d, _i]
+        L.oracle_line_sum_ref32.argtypes = [_i, _dp, _dp, _dp, _dp, _dp, _dp, _i, _ip, _dp, _dp,
+                                            _dp, _dp, _dp, _d, _d, _d, _i, _fp]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """The reference's own sources, compiled (raises if not built here)."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError(f"reference build missing: {REF_SO} (run `make -C oracle ref`)")
+        R = ctypes.CDLL(REF_SO)
+        R.ref_barnett_loop.argtypes = [_d, _d, _d, _dp, _dp, _dp, _i, _d, _d, _d, _d, _d, _d, _dp]
+        R.ref_barnett_loop_f.argtypes = [_f, _f, _f, _fp, _fp, _fp, _i, _f, _f, _f, _f, _f, _f, _fp]
+        R.ref_kroupa_pair.argtypes = [_d, _d, _d, _dp, _d, _d, _dp]
+        R.ref_kroupa_k1.restype = _d
+        R.ref_kroupa_cpu_line.argtypes = [_i, _dp, _dp, _dp, _dp, _dp, _dp, _i, _ip, _dp, _dp, _dp,
+                                          _dp, _dp, _d, _d, _d, _dp]
+        R.ref_kroupa_cpu_loopon.argtypes = [_i, _dp, _dp, _dp, _dp, _dp, _dp, _i, _ip, _dp, _dp,
+                                            _dp, _dp, _d, _d, _d, _dp]
+        R.ref_loop_polyline.argtypes = [_i, _dp, _dp, _dp, _dp, _d, _d, _dp, _i]
+        R.ref_loop_polyline.restype = _i
+        R.ref_loop_phys.argtypes = [_i, _dp, _d, _dp, _dp, _dp, _d, _dp, _dp]
+        R.ref_loop_loopons.argtypes = [_i, _dp, _d, _dp, _dp, _dp, _d, _dp, _i]
+        R.ref_loop_loopons.restype = _i
+        R.ref_regular_polygon.argtypes = [_i, _d, _dp, _dp, _dp]
+        R.ref_snap_mid_plane.argtypes = [_dp, _dp, _d, _i, _i, _i, _i, _dp, _dp]
+        R.ref_lattice.argtypes = [_i, _i, _i, _i, _d, _dp, _dp, _dp, _i]
+        R.ref_lattice.restype = _i
+        _ref = R
+    return _ref
+
+
+def threads() -> int:
+    return lib().oracle_threads()
+
+
+def k1() -> float:
+    return lib().oracle_k1()
+
+
+# ---------------------------------------------------------------- pair terms
+def _as3(v):
+    return np.ascontiguousarray(v, dtype=np.float64)
+
+
+def barnett_loop(p, V, c, b, use_ref=False):
+    """Field at p of the closed polyline V (nv x 3), apex c, Burgers b (FP64)."""
+    V = np.ascontiguousarray(np.asarray(V, dtype=np.float64).T)
+    u = np.zeros(3)
+    fn = ref().ref_barnett_loop if use_ref else lib().oracle_barnett_loop
+    fn(float(p[0]), float(p[1]), float(p[2]), _ptr(V[0]), _ptr(V[1]), _ptr(V[2]), V.shape[1],
+       float(c[0]), float(c[1]), float(c[2]), float(b[0]), float(b[1]), float(b[2]), _ptr(u))
+    return u
+
+
+def barnett_loop_f(p, V, c, b, use_ref=False):
+    """The float instantiation (R = float) of the same evaluation."""
+    V = np.ascontiguousarray(np.asarray(V, dtype=np.float32).T)
+    u = np.zeros(3, dtype=np.float32)
+    fn = ref().ref_barnett_loop_f if use_ref else lib().oracle_barnett_loop_f
+    fn(*[float(np.float32(t)) for t in p], _ptr(V[0], _fp), _ptr(V[1], _fp), _ptr(V[2], _fp),
+       V.shape[1], *[float(np.float32(t)) for t in c], *[float(np.float32(t)) for t in b],
+       _ptr(u, _fp))
+    return u
+
+
+def kroupa_pair(rho, phys, use_ref=False):
+    """One loopon image at separation rho; phys = (b3, m3, n3, dS)."""
+    ph = _as3(phys)
+    kk = k1()
+    b_n = float(ph[3] * ph[6] + ph[4] * ph[7] + ph[5] * ph[8])
+    u = np.zeros(3)
+    fn = ref().ref_kroupa_pair if use_ref else lib().oracle_kroupa_pair
+    fn(float(rho[0]), float(rho[1]), float(rho[2]), _ptr(ph), kk, b_n, _ptr(u))
+    return u
+
+
+# ---------------------------------------------------------------- packed sums
+def pack_line(loops):
+    """(voff, vx, vy, vz, c, b, phys) arrays from a list of line LoopData."""
+    voff = np.zeros(len(loops) + 1, dtype=np.int32)
+    for i, lp in enumerate(loops):
+        voff[i + 1] = voff[i] + len(lp.vx)
+    vx = np.concatenate([np.asarray(lp.vx, np.float64) for lp in loops]) if loops else np.zeros(0)
+    vy = np.concatenate([np.asarray(lp.vy, np.float64) for lp in loops]) if loops else np.zeros(0)
+    vz = np.concatenate([np.asarray(lp.vz, np.float64) for lp in loops]) if loops else np.zeros(0)
+    c = np.array([[lp.cx, lp.cy, lp.cz] for lp in loops], np.float64).reshape(-1)
+    b = np.array([lp.burgers() for lp in loops], np.float64).reshape(-1)
+    phys = np.array([lp.phys for lp in loops], np.float64).reshape(-1)
+    return voff, vx, vy, vz, c, b, phys
+
+
+def pack_loopon(loops):
+    poff = np.zeros(len(loops) + 1, dtype=np.int32)
+    for i, lp in enumerate(loops):
+        poff[i + 1] = poff[i] + len(lp.x)
+    px = np.concatenate([np.asarray(lp.x, np.float64) for lp in loops])
+    py = np.concatenate([np.asarray(lp.y, np.float64) for lp in loops])
+    pz = np.concatenate([np.asarray(lp.z, np.float64) for lp in loops])
+    phys = np.array([lp.phys for lp in loops], np.float64).reshape(-1)
+    return poff, px, py, pz, phys
+
+
+def _atoms(x, y, z):
+    return (np.ascontiguousarray(x, np.float64), np.ascontiguousarray(y, np.float64),
+            np.ascontiguousarray(z, np.float64))
+
+
+def line_sum(x, y, z, loops, box, img=1, u=None):
+    """Sum over loops x images (+-img) of the line field, FP64 (returns ux, uy, uz)."""
+    x, y, z = _atoms(x, y, z)
+    n = x.shape[0]
+    ux, uy, uz = (np.zeros(n), np.zeros(n), np.zeros(n)) if u is None else u
+    voff, vx, vy, vz, c, b, _ = pack_line(loops)
+    lib().oracle_line_sum(n, _ptr(x), _ptr(y), _ptr(z), _ptr(ux), _ptr(uy), _ptr(uz),
+                          len(loops), _ptr(voff, _ip), _ptr(vx), _ptr(vy), _ptr(vz), _ptr(c),
+                          _ptr(b), float(box[0]), float(box[1]), float(box[2]), int(img))
+    return ux, uy, uz
+
+
+def line_sum_ref32(x, y, z, loops, box, img=1):
+    """The reference's --fp32 line scheme restated on the CPU (yardstick)."""
+    x, y, z = _atoms(x, y, z)
+    n = x.shape[0]
+    ux, uy, uz = np.zeros(n), np.zeros(n), np.zeros(n)
+    voff, vx, vy, vz, c, b, _ = pack_line(loops)
+    scratch = np.zeros(3 * max(1, int(np.max(np.diff(voff))) if len(loops) else 1), np.float32)
+    lib().oracle_line_sum_ref32(n, _ptr(x), _ptr(y), _ptr(z), _ptr(ux), _ptr(uy), _ptr(uz),
+                                len(loops), _ptr(voff, _ip), _ptr(vx), _ptr(vy), _ptr(vz),
+                                _ptr(c), _ptr(b), float(box[0]), float(box[1]), float(box[2]),
+                                int(img), _ptr(scratch, _fp))
+    return ux, uy, uz
+
+
+def loopon_sum(x, y, z, loops, box, img=1, u=None):
+    """Sum over loops x images (+-img) x loopons of the Kroupa term, FP64."""
+    x, y, z = _atoms(x, y, z)
+    n = x.shape[0]
+    ux, uy, uz = (np.zeros(n), np.zeros(n), np.zeros(n)) if u is None else u
+    poff, px, py, pz, phys = pack_loopon(loops)
+    lib().oracle_loopon_sum(n, _ptr(x), _ptr(y), _ptr(z), _ptr(ux), _ptr(uy), _ptr(uz),
+                            len(loops), _ptr(poff, _ip), _ptr(px), _ptr(py), _ptr(pz),
+                            _ptr(phys), float(box[0]), float(box[1]), float(box[2]), int(img))
+    return ux, uy, uz
+
+
+# ---------------------------------------------------------------- reference calls
+def ref_kroupa_cpu_line(x, y, z, loops, box):
+    """The reference's kroupaCPU (line loops, +-1 images)."""
+    x, y, z = _atoms(x, y, z)
+    n = x.shape[0]
+    ux, uy, uz = np.zeros(n), np.zeros(n), np.zeros(n)
+    voff, vx, vy, vz, c, _, phys = pack_line(loops)
+    sec = np.zeros(len(loops))
+    ref().ref_kroupa_cpu_line(n, _ptr(x), _ptr(y), _ptr(z), _ptr(ux), _ptr(uy), _ptr(uz),
+                              len(loops), _ptr(voff, _ip), _ptr(vx), _ptr(vy), _ptr(vz), _ptr(c),
+                              _ptr(phys), float(box[0]), float(box[1]), float(box[2]), _ptr(sec))
+    return ux, uy, uz, sec
+
+
+def ref_kroupa_cpu_loopon(x, y, z, loops, box):
+    """The reference's kroupaCPU (loopon loops, +-1 images)."""
+    x, y, z = _atoms(x, y, z)
+    n = x.shape[0]
+    ux, uy, uz = np.zeros(n), np.zeros(n), np.zeros(n)
+    poff, px, py, pz, phys = pack_loopon(loops)
+    sec = np.zeros(len(loops))
+    ref().ref_kroupa_cpu_loopon(n, _ptr(x), _ptr(y), _ptr(z), _ptr(ux), _ptr(uy), _ptr(uz),
+                                len(loops), _ptr(poff, _ip), _ptr(px), _ptr(py), _ptr(pz),
+                                _ptr(phys), float(box[0]), float(box[1]), float(box[2]), _ptr(sec))
+    return ux, uy, uz, sec
+
+
+SHAPE_CODE = {"circle": 0, "rect": 1, "parallelogram": 2, "polygon": 3}
+
+
+def ref_polyline(shape, params, c, m, n, a0, eps, cap=4096):
+    out = np.zeros(3 * cap)
+    p = _as3(params)
+    nv = ref().ref_loop_polyline(SHAPE_CODE[shape], _ptr(p), _ptr(_as3(c)), _ptr(_as3(m)),
+                                 _ptr(_as3(n)), a0, eps, _ptr(out), cap)
+    return out[:3 * min(nv, cap)].reshape(-1, 3)
+
+
+def ref_loop_phys(shape, params, da, c, m, n, a0):
+    phys = np.zeros(10)
+    area = np.zeros(1)
+    ref().ref_loop_phys(SHAPE_CODE[shape], _ptr(_as3(params)), da, _ptr(_as3(c)), _ptr(_as3(m)),
+                        _ptr(_as3(n)), a0, _ptr(phys), _ptr(area))
+    return phys, float(area[0])
+
+
+def ref_loopons(shape, params, da, c, m, n, a0, cap=200000):
+    out = np.zeros(3 * cap)
+    npn = ref().ref_loop_loopons(SHAPE_CODE[shape], _ptr(_as3(params)), da, _ptr(_as3(c)),
+                                 _ptr(_as3(m)), _ptr(_as3(n)), a0, _ptr(out), cap)
+    return out[:3 * min(npn, cap)].reshape(-1, 3)
+
+
+def ref_regular_polygon(sides, R, m, n):
+    out = np.zeros(3 * sides)
+    ref().ref_regular_polygon(sides, R, _ptr(_as3(m)), _ptr(_as3(n)), _ptr(out))
+    return out.reshape(-1, 3)
+
+
+def ref_snap(center, n, a0, nx, ny, nz, lattice):
+    out = np.zeros(3)
+    sp = np.zeros(1)
+    ref().ref_snap_mid_plane(_ptr(_as3(center)), _ptr(_as3(n)), a0, nx, ny, nz,
+                             1 if lattice == "FCC" else 0, _ptr(out), _ptr(sp))
+    return out, float(sp[0])
+
+
+def ref_lattice(lattice, nx, ny, nz, a0):
+    cap = (4 if lattice == "FCC" else 2) * nx * ny * nz
+    x, y, z = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+    ref().ref_lattice(1 if lattice == "FCC" else 0, nx, ny, nz, a0, _ptr(x), _ptr(y), _ptr(z), cap)
+    return x, y, z

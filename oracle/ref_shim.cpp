@@ -1,0 +1,215 @@
+// ref_shim.cpp -- C entry points onto the reference's own hot-path sources.
+//
+// TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
+// the reference's Code/backend_cpu.cpp and Code/loop.cpp, where they lie, into
+// oracle/_ref/libdisco_ref.so.  Nothing of the reference is copied: this shim
+// only includes its headers and calls its functions, so the golden vectors and
+// the oracle's pins come from the reference's compiled code itself.
+//
+//   ref_barnett_loop      -> barnettLoop<double>   (Code/barnett.hpp:44-107)
+//   ref_barnett_loop_f    -> barnettLoop<float>    (Code/barnett.hpp:44-107)
+//   ref_kroupa_pair       -> kroupaPair            (Code/kroupa.hpp:75-80)
+//   ref_kroupa_cpu_line   -> kroupaCPU, line loops (Code/backend_cpu.cpp:28-63)
+//   ref_kroupa_cpu_loopon -> kroupaCPU, loopons    (Code/backend_cpu.cpp:64-99)
+//   ref_loop_*            -> class loop, regularPolygon, snapToMidPlane
+//                            (Code/loop.cpp:64-244,253-316,426-435,536-608,342-374)
+//   ref_lattice           -> generateLattice       (Code/lattice.hpp:35-51)
+#include <cstring>
+#include <iostream>
+#include <sstream>
+#include <vector>
+
+#include "backend.hpp"
+#include "barnett.hpp"
+#include "loop.hpp"
+
+using std::vector;
+
+namespace {
+vector<double> v3(const double* p) { return { p[0], p[1], p[2] }; }
+
+// Silences the reference's progress lines on stdout while fn runs.
+template <typename F> void quiet(F fn)
+{
+    std::ostringstream sink;
+    std::streambuf* old = std::cout.rdbuf(sink.rdbuf());
+    fn();
+    std::cout.rdbuf(old);
+}
+
+// Shape codes for ref_loop_make
+enum { SHAPE_CIRCLE = 0, SHAPE_RECT = 1, SHAPE_PARALLELOGRAM = 2, SHAPE_POLYGON = 3 };
+
+// Builds a loop; params by shape:
+//   circle:        p[0] = radius
+//   rect:          p[0] = half_m, p[1] = half_p, p[2] = corner_r
+//   parallelogram: p[0] = half1, p[1] = half2, p[2..4] = dir1, p[5..7] = dir2
+//   polygon:       p[0] = sides, p[1] = circumradius R, p[2] = corner_r
+//                  (vertices from regularPolygon(sides, R, {}, m, n))
+loop make_loop(int shape, const double* p, double da, const double* c, const double* m,
+               const double* n, double a0)
+{
+    const vector<double> one = { 1., 1., 1. };
+    switch (shape) {
+        case SHAPE_CIRCLE:
+            return loop(da, p[0], v3(c), v3(m), v3(n), a0, one);
+        case SHAPE_RECT:
+            return loop(da, p[0], p[1], p[2], v3(c), v3(m), v3(n), a0, one);
+        case SHAPE_PARALLELOGRAM:
+            return loop(da, p[0], p[1], v3(p + 2), v3(p + 5), v3(c), v3(m), v3(n), a0, one);
+        default:
+            return loop(da, regularPolygon((int)p[0], p[1], {}, v3(m), v3(n)), p[2], v3(c),
+                        v3(m), v3(n), a0, one);
+    }
+}
+
+void build_line_loops(int nloops, const int* voff, const double* vx, const double* vy,
+                      const double* vz, const double* c, const double* phys,
+                      vector<LoopData>& loops)
+{
+    loops.resize(nloops);
+    for (int l = 0; l < nloops; l++) {
+        LoopData& d = loops[l];
+        const double* ph = phys + 10 * l;
+        d.phys = { ph[0], ph[1], ph[2], ph[3], ph[4], ph[5], ph[6], ph[7], ph[8], ph[9] };
+        d.line = true;
+        d.vx.assign(vx + voff[l], vx + voff[l + 1]);
+        d.vy.assign(vy + voff[l], vy + voff[l + 1]);
+        d.vz.assign(vz + voff[l], vz + voff[l + 1]);
+        d.cx = c[3 * l]; d.cy = c[3 * l + 1]; d.cz = c[3 * l + 2];
+    }
+}
+}  // namespace
+
+extern "C" {
+
+void ref_barnett_loop(double px, double py, double pz, const double* vx, const double* vy,
+                      const double* vz, int nv, double cx, double cy, double cz, double bx,
+                      double by, double bz, double* u)
+{
+    barnettLoop<double>(px, py, pz, vx, vy, vz, nv, cx, cy, cz, bx, by, bz, u[0], u[1], u[2]);
+}
+
+void ref_barnett_loop_f(float px, float py, float pz, const float* vx, const float* vy,
+                        const float* vz, int nv, float cx, float cy, float cz, float bx,
+                        float by, float bz, float* u)
+{
+    barnettLoop<float>(px, py, pz, vx, vy, vz, nv, cx, cy, cz, bx, by, bz, u[0], u[1], u[2]);
+}
+
+void ref_kroupa_pair(double rx, double ry, double rz, const double* ph, double k1, double b_n,
+                     double* u)
+{
+    const LoopPhys L = { ph[0], ph[1], ph[2], ph[3], ph[4], ph[5], ph[6], ph[7], ph[8], ph[9] };
+    kroupaPair(rx, ry, rz, L, k1, b_n, u[0], u[1], u[2]);
+}
+
+double ref_kroupa_k1() { return kroupaK1(); }
+
+// kroupaCPU on line loops; phys[10 l ..] is the loop's LoopPhys (b = phys.b * phys.m)
+void ref_kroupa_cpu_line(int n, const double* x, const double* y, const double* z, double* ux,
+                         double* uy, double* uz, int nloops, const int* voff, const double* vx,
+                         const double* vy, const double* vz, const double* c,
+                         const double* phys, double Lx, double Ly, double Lz,
+                         double* loop_sec)
+{
+    vector<LoopData> loops;
+    build_line_loops(nloops, voff, vx, vy, vz, c, phys, loops);
+    const AtomArrays atoms = { x, y, z, ux, uy, uz, n };
+    BackendTiming t;
+    quiet([&] { kroupaCPU(atoms, loops, Lx, Ly, Lz, t); });
+    if (loop_sec)
+        for (int l = 0; l < nloops; l++) loop_sec[l] = t.loopSec[l];
+}
+
+void ref_kroupa_cpu_loopon(int n, const double* x, const double* y, const double* z,
+                           double* ux, double* uy, double* uz, int nloops, const int* poff,
+                           const double* px, const double* py, const double* pz,
+                           const double* phys, double Lx, double Ly, double Lz,
+                           double* loop_sec)
+{
+    vector<LoopData> loops(nloops);
+    for (int l = 0; l < nloops; l++) {
+        LoopData& d = loops[l];
+        const double* ph = phys + 10 * l;
+        d.phys = { ph[0], ph[1], ph[2], ph[3], ph[4], ph[5], ph[6], ph[7], ph[8], ph[9] };
+        d.x.assign(px + poff[l], px + poff[l + 1]);
+        d.y.assign(py + poff[l], py + poff[l + 1]);
+        d.z.assign(pz + poff[l], pz + poff[l + 1]);
+    }
+    const AtomArrays atoms = { x, y, z, ux, uy, uz, n };
+    BackendTiming t;
+    quiet([&] { kroupaCPU(atoms, loops, Lx, Ly, Lz, t); });
+    if (loop_sec)
+        for (int l = 0; l < nloops; l++) loop_sec[l] = t.loopSec[l];
+}
+
+// Polyline of a loop (Code/loop.cpp:536-580) at tolerance eps.  Returns the
+// vertex count; writes up to cap vertices into out (x0 y0 z0 x1 ...).
+int ref_loop_polyline(int shape, const double* p, const double* c, const double* m,
+                      const double* n, double a0, double eps, double* out, int cap)
+{
+    const loop lp = make_loop(shape, p, 1.0, c, m, n, a0);
+    const vector<vector<double>> P = lp.polyline(eps);
+    for (int k = 0; k < (int)P.size() && k < cap; k++)
+        for (int i = 0; i < 3; i++) out[3 * k + i] = P[k][i];
+    return (int)P.size();
+}
+
+// Loop physics as the driver assembles it (Code/main.cpp:698-699):
+// phys = {|b| x3, m_hat, n_hat, dS} after generateSurface(); area = contour area.
+void ref_loop_phys(int shape, const double* p, double da, const double* c, const double* m,
+                   const double* n, double a0, double* phys, double* area)
+{
+    loop lp = make_loop(shape, p, da, c, m, n, a0);
+    lp.generateSurface();
+    const vector<double> b = lp.getBurgers(), mh = lp.getM(), nh = lp.getN();
+    const double out[10] = { b[0], b[1], b[2], mh[0], mh[1], mh[2], nh[0], nh[1], nh[2],
+                             lp.getSurface() };
+    std::memcpy(phys, out, sizeof(out));
+    *area = lp.getContourArea();
+}
+
+// Loopon grid (Code/loop.cpp:253-273).  Returns the count; writes up to cap.
+int ref_loop_loopons(int shape, const double* p, double da, const double* c, const double* m,
+                     const double* n, double a0, double* out, int cap)
+{
+    loop lp = make_loop(shape, p, da, c, m, n, a0);
+    lp.generateSurface();
+    lp.generateLoopons();
+    const int np = lp.getNumberOfLoopons();
+    for (int k = 0; k < np && k < cap; k++) {
+        const vector<double> q = lp.getOneLoopon(k);
+        for (int i = 0; i < 3; i++) out[3 * k + i] = q[i];
+    }
+    return np;
+}
+
+// regularPolygon (Code/loop.cpp:585-608), default vertex axis
+void ref_regular_polygon(int sides, double R, const double* m, const double* n, double* out)
+{
+    const vector<vector<double>> P = regularPolygon(sides, R, {}, v3(m), v3(n));
+    for (int k = 0; k < sides; k++)
+        for (int i = 0; i < 3; i++) out[3 * k + i] = P[k][i];
+}
+
+// snapToMidPlane (Code/loop.cpp:342-374); lattice 0 = BCC, 1 = FCC
+void ref_snap_mid_plane(const double* center, const double* n, double a0, int nx, int ny,
+                        int nz, int lattice, double* out, double* spacing)
+{
+    const vector<double> s = snapToMidPlane(v3(center), v3(n), a0, nx, ny, nz, *spacing,
+                                            lattice ? Lattice::FCC : Lattice::BCC);
+    for (int i = 0; i < 3; i++) out[i] = s[i];
+}
+
+// generateLattice (Code/lattice.hpp:35-51) into SoA arrays of capacity cap
+int ref_lattice(int lattice, int nx, int ny, int nz, double a0, double* x, double* y, double* z,
+                int cap)
+{
+    const vector<vector<double>> A =
+        generateLattice(lattice ? Lattice::FCC : Lattice::BCC, nx, ny, nz, a0);
+    for (int i = 0; i < (int)A.size() && i < cap; i++) { x[i] = A[i][0]; y[i] = A[i][1]; z[i] = A[i][2]; }
+    return (int)A.size();
+}
+
+}  // extern "C"
