@@ -95,6 +95,15 @@ class _Timing(ctypes.Structure):
                 ("loop_sec", ctypes.POINTER(ctypes.c_double))]
 
 
+class _Sample(ctypes.Structure):
+    _fields_ = [("atom_offset", ctypes.c_longlong), ("n", ctypes.c_int), ("nloops", ctypes.c_int),
+                ("loops", ctypes.POINTER(_LineLoop)), ("L", ctypes.c_double * 3)]
+
+
+class _Tuning(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int), ("tile", ctypes.c_int), ("groups", ctypes.c_int)]
+
+
 class _DeviceInfo(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 256), ("compute_units", ctypes.c_int),
                 ("global_mem", ctypes.c_size_t), ("lds_per_block", ctypes.c_size_t),
@@ -150,6 +159,16 @@ def lib():
         L.disco_hip_plan_evals_per_atom.argtypes = [_vp]
         L.disco_hip_plan_evals_per_atom.restype = _d
         L.disco_hip_plan_free.argtypes = [_vp]
+        L.disco_hip_batch_plan.argtypes = [_vp, ctypes.POINTER(_Sample), _i, _i, _i,
+                                           ctypes.POINTER(_vp)]
+        L.disco_hip_batch_set_tuning.argtypes = [_vp, ctypes.POINTER(_Tuning)]
+        L.disco_hip_batch_get_tuning.argtypes = [_vp, ctypes.POINTER(_Tuning)]
+        L.disco_hip_batch_autotune.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp,
+                                               ctypes.POINTER(_Tuning), _dp]
+        L.disco_hip_batch_run.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]
+        L.disco_hip_batch_evals.argtypes = [_vp]
+        L.disco_hip_batch_evals.restype = _d
+        L.disco_hip_batch_free.argtypes = [_vp]
         _lib = L
     return _lib
 
@@ -351,6 +370,85 @@ class Plan:
     def free(self):
         if self._h:
             lib().disco_hip_plan_free(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many samples summed by one persistent launch per pass, replayed as a
+    HIP graph (disco_hip_batch_*).  `samples` is a list of geometry.Sample
+    (or any object with .n, .loops, .box) laid out back to back in the atom
+    arrays, in list order."""
+
+    def __init__(self, ctx, samples, img=1, prec=PREC_FP32C):
+        self.ctx = ctx
+        self._h = _vp()
+        arr = (_Sample * max(1, len(samples)))()
+        self._keep = []
+        off = 0
+        self.offsets = []
+        for i, s in enumerate(samples):
+            la, keep = line_structs(s.loops)
+            self._keep += [la, keep]
+            arr[i].atom_offset = off
+            arr[i].n = int(s.n)
+            arr[i].nloops = len(s.loops)
+            arr[i].loops = ctypes.cast(la, ctypes.POINTER(_LineLoop))
+            arr[i].L[:] = [float(t) for t in s.box]
+            self.offsets.append(off)
+            off += int(s.n)
+        self.n = off
+        ctx.check(lib().disco_hip_batch_plan(ctx.handle, arr, len(samples), int(img), int(prec),
+                                             ctypes.byref(self._h)))
+        self._keep = None          # the plan copied everything it needs
+
+    def evals(self) -> float:
+        return lib().disco_hip_batch_evals(self._h)
+
+    @property
+    def tuning(self):
+        t = _Tuning()
+        self.ctx.check(lib().disco_hip_batch_get_tuning(self._h, ctypes.byref(t)))
+        return {"block": t.block, "tile": t.tile, "groups": t.groups}
+
+    def set_tuning(self, block, tile, groups):
+        t = _Tuning(block, tile, groups)
+        self.ctx.check(lib().disco_hip_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    @staticmethod
+    def _ptrs(ts):
+        import torch
+        for t in ts:
+            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+                raise ValueError("device arrays must be contiguous float64 CUDA tensors")
+        return [t.data_ptr() for t in ts]
+
+    def autotune(self, x, y, z, ux, uy, uz, stride=8, stream=None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        t = _Tuning()
+        ms = ctypes.c_double(0.0)
+        self.ctx.check(lib().disco_hip_batch_autotune(self._h, *self._ptrs((x, y, z, ux, uy, uz)),
+                                                      int(stride), stream, ctypes.byref(t),
+                                                      ctypes.byref(ms)))
+        return {"block": t.block, "tile": t.tile, "groups": t.groups}, ms.value
+
+    def run_torch(self, x, y, z, ux, uy, uz, accumulate=False, stream=None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        self.ctx.check(lib().disco_hip_batch_run(self._h, *self._ptrs((x, y, z, ux, uy, uz)),
+                                                 1 if accumulate else 0, stream))
+
+    def free(self):
+        if self._h:
+            lib().disco_hip_batch_free(self._h)
             self._h = _vp()
 
     def __del__(self):

@@ -1,0 +1,45 @@
+// ctx_internal.h -- the device context shared by the library's sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <string>
+
+#include "disco_hip.h"
+
+struct disco_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    // cached atom buffers (grow-only)
+    double* d_atoms = nullptr;   // x | y | z | ux | uy | uz
+    size_t cap_atoms = 0;
+};
+
+namespace disco {
+
+inline int fail(disco_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+inline double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+}  // namespace disco
+
+// Returns from the calling function with DISCO_ERR_HIP (or DISCO_ERR_OOM) and
+// records the failing call on the context.
+#define DISCO_HIP_TRY(ctx, call)                                                        \
+    do {                                                                                \
+        hipError_t _e = (call);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return disco::fail((ctx), _e == hipErrorOutOfMemory ? DISCO_ERR_OOM : DISCO_ERR_HIP, \
+                               std::string(#call) + ": " + hipGetErrorString(_e));      \
+    } while (0)

@@ -17,7 +17,9 @@
 // (:302-373) and kroupa_kernel* (:63-291).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -27,6 +29,7 @@
 #include <vector>
 
 #define DISCO_HD __device__ __forceinline__
+#include "ctx_internal.h"
 #include "disco_hip.h"
 #include "pair_terms.h"
 
@@ -60,9 +63,9 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kLooponChunk = 64;   // loopons per FP32 partial sum (flushed to FP64)
 
-// ------------------------------------------------------------------ kernels
+// ------------------------------------------------------------------ per-atom sums
 // Near predicate of the FP32C mode: the loop image lies within the loop's near
-// slab (|h| < hn and in-plane radius^2 < rn2).  Both line kernels evaluate it
+// slab (|h| < hn and in-plane radius^2 < rn2).  Both line passes evaluate it
 // with contraction off, so they take exactly the same decision for every
 // (atom, loop, image) and each triple is summed by exactly one of them.
 __device__ __forceinline__ bool near_slab(double qx, double qy, double qz, const LineLoop& L)
@@ -73,115 +76,74 @@ __device__ __forceinline__ bool near_slab(double qx, double qy, double qz, const
     return fabs(h) < L.hn && r2 < L.rn2;
 }
 
-// FP64 line sum: every (loop, image) in FP64.
-__global__ __launch_bounds__(kBlock) void line_fp64_kernel(
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
-    const LineLoop* __restrict__ loops, int nloops, const SegD* __restrict__ segd, double Lx,
-    double Ly, double Lz, int img, int accumulate)
-{
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const double px = x[i], py = y[i], pz = z[i];
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int l = 0; l < nloops; l++) {
-        const LineLoop& L = loops[l];
-        const double pcx = px - L.cx, pcy = py - L.cy, pcz = pz - L.cz;
-        for (int di = -img; di <= img; di++)
-            for (int dj = -img; dj <= img; dj++)
-                for (int dk = -img; dk <= img; dk++) {
-                    double ox = 0.0, oy = 0.0, oz = 0.0;
-                    line_image_f64<DevMath>(pcx - di * Lx, pcy - dj * Ly, pcz - dk * Lz,
-                                            segd + L.seg0, L.nv, L.bx, L.by, L.bz, ox, oy, oz);
-                    sx += ox; sy += oy; sz += oz;
-                }
-    }
-    if (accumulate) { sx += ux[i]; sy += uy[i]; sz += uz[i]; }
-    ux[i] = sx; uy[i] = sy; uz[i] = sz;
-}
+// The three line passes over loops [l0, l1) and images [-img, img]^3 (kk
+// fastest) for the atom at p; loop, segment and image data are wave-uniform.
+enum { PASS_FP64 = 0, PASS_FAR = 1, PASS_NEAR = 2 };
 
-// FP32C far pass: every (loop, image) evaluated in FP32 by every lane (the
-// segment loop stays in wave-uniform control flow, so the tables are scalar
-// loads); the image field is added in FP64 unless the lane is near.
-__global__ __launch_bounds__(kBlock) void line_far_f32_kernel(
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
-    const LineLoop* __restrict__ loops, int nloops, const SegF* __restrict__ segf, double Lx,
-    double Ly, double Lz, int img, int accumulate)
+template <int PASS>
+__device__ __forceinline__ bool atom_line(double px, double py, double pz,
+                                          const LineLoop* __restrict__ loops, int l0, int l1,
+                                          const SegF* __restrict__ segf,
+                                          const SegD* __restrict__ segd, double Lx, double Ly,
+                                          double Lz, int img, int inner, double& sx, double& sy,
+                                          double& sz)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const double px = x[i], py = y[i], pz = z[i];
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int l = 0; l < nloops; l++) {
-        const LineLoop& L = loops[l];
-        const double pcx = px - L.cx, pcy = py - L.cy, pcz = pz - L.cz;
-        const float bx = (float)L.bx, by = (float)L.by, bz = (float)L.bz;
-        for (int di = -img; di <= img; di++)
-            for (int dj = -img; dj <= img; dj++)
-                for (int dk = -img; dk <= img; dk++) {
-                    const double qx = pcx - di * Lx, qy = pcy - dj * Ly, qz = pcz - dk * Lz;
-                    float ox = 0.f, oy = 0.f, oz = 0.f;
-                    line_image_f32<DevMath>((float)qx, (float)qy, (float)qz, segf + L.seg0,
-                                            L.nv, bx, by, bz, ox, oy, oz);
-                    if (!near_slab(qx, qy, qz, L)) {
-                        sx += (double)ox; sy += (double)oy; sz += (double)oz;
-                    }
-                }
-    }
-    if (accumulate) { sx += ux[i]; sy += uy[i]; sz += uz[i]; }
-    ux[i] = sx; uy[i] = sy; uz[i] = sz;
-}
-
-// FP32C near pass: adds the FP64 field of the near (loop, image) pairs.
-__global__ __launch_bounds__(kBlock) void line_near_f64_kernel(
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
-    const LineLoop* __restrict__ loops, int nloops, const SegD* __restrict__ segd, double Lx,
-    double Ly, double Lz, int img)
-{
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const double px = x[i], py = y[i], pz = z[i];
-    double sx = 0.0, sy = 0.0, sz = 0.0;
     bool any = false;
-    for (int l = 0; l < nloops; l++) {
+    for (int l = l0; l < l1; l++) {
         const LineLoop& L = loops[l];
         const double pcx = px - L.cx, pcy = py - L.cy, pcz = pz - L.cz;
         for (int di = -img; di <= img; di++)
             for (int dj = -img; dj <= img; dj++)
                 for (int dk = -img; dk <= img; dk++) {
+                    if (max(abs(di), max(abs(dj), abs(dk))) <= inner) continue;   // inner shell
                     const double qx = pcx - di * Lx, qy = pcy - dj * Ly, qz = pcz - dk * Lz;
-                    if (near_slab(qx, qy, qz, L)) {
+                    if (PASS == PASS_FP64) {
                         double ox = 0.0, oy = 0.0, oz = 0.0;
                         line_image_f64<DevMath>(qx, qy, qz, segd + L.seg0, L.nv, L.bx, L.by,
                                                 L.bz, ox, oy, oz);
                         sx += ox; sy += oy; sz += oz;
-                        any = true;
+                    } else if (PASS == PASS_FAR) {
+                        // every lane evaluates (uniform control flow keeps the
+                        // segment tables in scalar loads); near lanes discard it
+                        float ox = 0.f, oy = 0.f, oz = 0.f;
+                        line_image_f32<DevMath>((float)qx, (float)qy, (float)qz, segf + L.seg0,
+                                                L.nv, (float)L.bx, (float)L.by, (float)L.bz,
+                                                ox, oy, oz);
+                        if (!near_slab(qx, qy, qz, L)) {
+                            sx += (double)ox; sy += (double)oy; sz += (double)oz;
+                        }
+                    } else {
+                        if (near_slab(qx, qy, qz, L)) {
+                            double ox = 0.0, oy = 0.0, oz = 0.0;
+                            line_image_f64<DevMath>(qx, qy, qz, segd + L.seg0, L.nv, L.bx,
+                                                    L.by, L.bz, ox, oy, oz);
+                            sx += ox; sy += oy; sz += oz;
+                            any = true;
+                        }
                     }
                 }
     }
-    if (any) { ux[i] += sx; uy[i] += sy; uz[i] += sz; }
+    return PASS == PASS_NEAR ? any : true;
 }
 
+// Kroupa sum over loops [l0, l1); FP32: partial sums of 64 loopons in float,
+// flushed to FP64.
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void loopon_sum_kernel(
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
-    const LooponLoop* __restrict__ loops, int nloops, const float4* __restrict__ pf,
-    const double4* __restrict__ pd, double Lx, double Ly, double Lz, int img, int accumulate)
+__device__ __forceinline__ void atom_loopon(double px, double py, double pz,
+                                            const LooponLoop* __restrict__ loops, int l0, int l1,
+                                            const float4* __restrict__ pf,
+                                            const double4* __restrict__ pd, double Lx,
+                                            double Ly, double Lz, int img, int inner,
+                                            double& ax, double& ay, double& az)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const double px = x[i], py = y[i], pz = z[i];
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    for (int l = 0; l < nloops; l++) {
+    for (int l = l0; l < l1; l++) {
         const LooponLoop& L = loops[l];
         const double pcx = px - L.cx, pcy = py - L.cy, pcz = pz - L.cz;
         double s1 = 0.0, s2 = 0.0, s3x = 0.0, s3y = 0.0, s3z = 0.0;
         for (int di = -img; di <= img; di++)
             for (int dj = -img; dj <= img; dj++)
                 for (int dk = -img; dk <= img; dk++) {
+                    if (max(abs(di), max(abs(dj), abs(dk))) <= inner) continue;   // inner shell
                     const double qx = pcx - di * Lx, qy = pcy - dj * Ly, qz = pcz - dk * Lz;
                     if (PREC == DISCO_PREC_FP64) {
                         loopon_image_f64<DevMath>(qx, qy, qz, pd + L.p0, L.np, L.mx, L.my, L.mz,
@@ -205,6 +167,56 @@ __global__ __launch_bounds__(kBlock) void loopon_sum_kernel(
         ay += L.by * (L.ny * s1 + L.my * s2 + s3y);
         az += L.bz * (L.nz * s1 + L.mz * s2 + s3z);
     }
+}
+
+// ------------------------------------------------------------------ one-sample kernels
+// FP64 line sum (PASS_FP64) or the FP32C far pass (PASS_FAR): write u (or add).
+template <int PASS>
+__global__ __launch_bounds__(kBlock) void line_kernel(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
+    const LineLoop* __restrict__ loops, int nloops, const SegF* __restrict__ segf,
+    const SegD* __restrict__ segd, double Lx, double Ly, double Lz, int img, int inner,
+    int accumulate)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    atom_line<PASS>(x[i], y[i], z[i], loops, 0, nloops, segf, segd, Lx, Ly, Lz, img, inner, sx,
+                    sy, sz);
+    if (accumulate) { sx += ux[i]; sy += uy[i]; sz += uz[i]; }
+    ux[i] = sx; uy[i] = sy; uz[i] = sz;
+}
+
+// FP32C near pass: adds the FP64 field of the near (loop, image) pairs.
+__global__ __launch_bounds__(kBlock) void line_near_kernel(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
+    const LineLoop* __restrict__ loops, int nloops, const SegD* __restrict__ segd, double Lx,
+    double Ly, double Lz, int img, int inner)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    if (atom_line<PASS_NEAR>(x[i], y[i], z[i], loops, 0, nloops, nullptr, segd, Lx, Ly, Lz, img,
+                             inner, sx, sy, sz)) {
+        ux[i] += sx; uy[i] += sy; uz[i] += sz;
+    }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void loopon_kernel(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
+    const LooponLoop* __restrict__ loops, int nloops, const float4* __restrict__ pf,
+    const double4* __restrict__ pd, double Lx, double Ly, double Lz, int img, int inner,
+    int accumulate)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    atom_loopon<PREC>(x[i], y[i], z[i], loops, 0, nloops, pf, pd, Lx, Ly, Lz, img, inner, ax, ay,
+                      az);
     if (accumulate) { ax += ux[i]; ay += uy[i]; az += uz[i]; }
     ux[i] = ax; uy[i] = ay; uz[i] = az;
 }
@@ -218,24 +230,64 @@ __global__ __launch_bounds__(kBlock) void zero3_kernel(double* __restrict__ a,
     if (i < n) { a[i] = 0.0; b[i] = 0.0; c[i] = 0.0; }
 }
 
-double now_s()
+// ------------------------------------------------------------------ batched samples
+// One sample of a batch: its atoms [a0, a0 + n) of the concatenated arrays,
+// its loops [l0, l1) of the concatenated loop table, its box.
+struct alignas(64) BatchSample {
+    long long a0;
+    int n, l0, l1, pad;
+    double Lx, Ly, Lz;
+    double pad2[2];
+};
+// A work item: atoms [start, start + count) of one sample.
+struct alignas(16) BatchItem {
+    int sample, start, count, pad;
+};
+
+// Work-queue counters: one per pass, cleared by a kernel (no memset node in a
+// captured graph).
+__global__ void reset_counters_kernel(unsigned* __restrict__ c, int n)
 {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
-        .count();
+    if (threadIdx.x < n) c[threadIdx.x] = 0u;
+}
+
+// Persistent batched line pass: each workgroup takes items from the queue
+// (one vector atomic per item) until every item is taken; items are sorted by
+// decreasing cost on the host (longest first), so the tail is short.
+template <int BLOCK, int PASS>
+__global__ __launch_bounds__(BLOCK) void batch_line_kernel(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz,
+    const BatchSample* __restrict__ samples, const BatchItem* __restrict__ items, int nitems,
+    unsigned* __restrict__ counter, const LineLoop* __restrict__ loops,
+    const SegF* __restrict__ segf, const SegD* __restrict__ segd, int img, int inner,
+    int accumulate)
+{
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (it >= nitems) return;
+        const BatchItem I = items[it];
+        const BatchSample& S = samples[I.sample];
+        for (int a = threadIdx.x; a < I.count; a += BLOCK) {
+            const long long i = S.a0 + I.start + a;
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            const bool any = atom_line<PASS>(x[i], y[i], z[i], loops, S.l0, S.l1, segf, segd,
+                                             S.Lx, S.Ly, S.Lz, img, inner, sx, sy, sz);
+            if (PASS == PASS_NEAR) {
+                if (any) { ux[i] += sx; uy[i] += sy; uz[i] += sz; }
+            } else {
+                if (accumulate) { sx += ux[i]; sy += uy[i]; sz += uz[i]; }
+                ux[i] = sx; uy[i] = sy; uz[i] = sz;
+            }
+        }
+    }
 }
 
 }  // namespace
-
-// ------------------------------------------------------------------ context
-struct disco_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::string err;
-    // cached atom buffers (grow-only)
-    double* d_atoms = nullptr;   // x | y | z | ux | uy | uz
-    size_t cap_atoms = 0;
-};
 
 struct disco_plan {
     disco_ctx* ctx = nullptr;
@@ -243,6 +295,7 @@ struct disco_plan {
     int prec = 0;
     int nloops = 0;        // packed (non-degenerate) loops
     int img = 1;
+    int inner = -1;        // images with max(|i|,|j|,|k|) <= inner are skipped
     double L[3] = {0, 0, 0};
     double work = 0;       // segments (or loopons) per image
     void* d_loops = nullptr;
@@ -253,21 +306,30 @@ struct disco_plan {
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 };
 
+struct disco_batch {
+    disco_ctx* ctx = nullptr;
+    int prec = 0, img = 1, nsamples = 0;
+    int inner = -1;
+    std::vector<BatchSample> samples;     // host copy
+    std::vector<double> seg_per_sample;   // segments of each sample
+    double evals = 0;
+    disco_batch_tuning tuning = {256, 256, 0};
+    int nitems = 0;
+    void* d_loops = nullptr;
+    void* d_segf = nullptr;
+    void* d_segd = nullptr;
+    BatchSample* d_samples = nullptr;
+    BatchItem* d_items = nullptr;
+    size_t cap_items = 0;
+    unsigned* d_counters = nullptr;
+    hipStream_t cap_stream = nullptr;     // private stream for graph capture
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void* key[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    disco_batch_tuning key_tuning = {0, 0, 0};
+};
+
 namespace {
-
-int fail(disco_ctx* ctx, int code, const std::string& msg)
-{
-    if (ctx) ctx->err = msg;
-    return code;
-}
-
-#define DISCO_HIP_TRY(ctx, call)                                                        \
-    do {                                                                                \
-        hipError_t _e = (call);                                                         \
-        if (_e != hipSuccess)                                                           \
-            return fail((ctx), _e == hipErrorOutOfMemory ? DISCO_ERR_OOM : DISCO_ERR_HIP, \
-                        std::string(#call) + ": " + hipGetErrorString(_e));             \
-    } while (0)
 
 int check_common(disco_ctx* ctx, int n, double Lx, double Ly, double Lz, int img, int prec,
                  int nloops)
@@ -335,27 +397,29 @@ int launch(disco_plan* p, int n, const double* x, const double* y, const double*
         const SegF* sf = (const SegF*)p->d_f;
         const SegD* sd = (const SegD*)p->d_d;
         if (p->prec == DISCO_PREC_FP64) {
-            hipLaunchKernelGGL(line_fp64_kernel, grid, block, 0, s, x, y, z, ux, uy, uz, n, lp,
-                               p->nloops, sd, p->L[0], p->L[1], p->L[2], p->img, accumulate);
+            hipLaunchKernelGGL(line_kernel<PASS_FP64>, grid, block, 0, s, x, y, z, ux, uy, uz, n,
+                               lp, p->nloops, sf, sd, p->L[0], p->L[1], p->L[2], p->img,
+                               p->inner, accumulate);
         } else {
-            hipLaunchKernelGGL(line_far_f32_kernel, grid, block, 0, s, x, y, z, ux, uy, uz, n,
-                               lp, p->nloops, sf, p->L[0], p->L[1], p->L[2], p->img, accumulate);
+            hipLaunchKernelGGL(line_kernel<PASS_FAR>, grid, block, 0, s, x, y, z, ux, uy, uz, n,
+                               lp, p->nloops, sf, sd, p->L[0], p->L[1], p->L[2], p->img,
+                               p->inner, accumulate);
             if (prof) DISCO_HIP_TRY(ctx, hipEventRecord(p->ev[1], s));
-            hipLaunchKernelGGL(line_near_f64_kernel, grid, block, 0, s, x, y, z, ux, uy, uz, n,
-                               lp, p->nloops, sd, p->L[0], p->L[1], p->L[2], p->img);
+            hipLaunchKernelGGL(line_near_kernel, grid, block, 0, s, x, y, z, ux, uy, uz, n, lp,
+                               p->nloops, sd, p->L[0], p->L[1], p->L[2], p->img, p->inner);
         }
     } else {
         const LooponLoop* lp = (const LooponLoop*)p->d_loops;
         const float4* pf = (const float4*)p->d_f;
         const double4* pd = (const double4*)p->d_d;
         if (p->prec == DISCO_PREC_FP64)
-            hipLaunchKernelGGL(loopon_sum_kernel<DISCO_PREC_FP64>, grid, block, 0, s, x, y, z,
-                               ux, uy, uz, n, lp, p->nloops, pf, pd, p->L[0], p->L[1], p->L[2],
-                               p->img, accumulate);
+            hipLaunchKernelGGL(loopon_kernel<DISCO_PREC_FP64>, grid, block, 0, s, x, y, z, ux, uy,
+                               uz, n, lp, p->nloops, pf, pd, p->L[0], p->L[1], p->L[2], p->img,
+                               p->inner, accumulate);
         else
-            hipLaunchKernelGGL(loopon_sum_kernel<DISCO_PREC_FP32C>, grid, block, 0, s, x, y, z,
-                               ux, uy, uz, n, lp, p->nloops, pf, pd, p->L[0], p->L[1], p->L[2],
-                               p->img, accumulate);
+            hipLaunchKernelGGL(loopon_kernel<DISCO_PREC_FP32C>, grid, block, 0, s, x, y, z, ux,
+                               uy, uz, n, lp, p->nloops, pf, pd, p->L[0], p->L[1], p->L[2],
+                               p->img, p->inner, accumulate);
     }
     DISCO_HIP_TRY(ctx, hipGetLastError());
     if (prof) {
@@ -468,6 +532,131 @@ int host_sum(disco_ctx* ctx, disco_plan* p, const std::vector<double>& work, int
         }
     }
     return DISCO_OK;
+}
+
+// ------------------------------------------------------------------ batches
+void batch_release_graph(disco_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+void free_batch(disco_batch* b)
+{
+    if (!b) return;
+    batch_release_graph(b);
+    for (void* q : {b->d_loops, b->d_segf, b->d_segd, (void*)b->d_samples, (void*)b->d_items,
+                    (void*)b->d_counters})
+        if (q) (void)hipFree(q);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    delete b;
+}
+
+bool tuning_valid(const disco_batch_tuning& t)
+{
+    return (t.block == 64 || t.block == 128 || t.block == 256) && t.tile >= t.block &&
+           t.tile % t.block == 0 && t.groups >= 1;
+}
+
+// Work items of the given samples (every `stride`-th), tiles of `tile` atoms,
+// longest first.  Returns the item list and the evaluations it covers.
+std::vector<BatchItem> make_items(const disco_batch* b, int tile, int stride, double* evals)
+{
+    struct Keyed { double cost; BatchItem it; };
+    std::vector<Keyed> v;
+    const double o = 2.0 * b->img + 1.0;
+    const double ii = b->inner >= 0 ? 2.0 * b->inner + 1.0 : 0.0;
+    const double im = o * o * o - ii * ii * ii;
+    double ev = 0;
+    for (int k = 0; k < b->nsamples; k += stride) {
+        const BatchSample& S = b->samples[k];
+        if (S.l1 <= S.l0) continue;
+        for (int st = 0; st < S.n; st += tile) {
+            BatchItem it;
+            it.sample = k; it.start = st; it.count = std::min(tile, S.n - st); it.pad = 0;
+            const double c = (double)it.count * b->seg_per_sample[k] * im;
+            v.push_back({c, it});
+            ev += c;
+        }
+    }
+    std::stable_sort(v.begin(), v.end(), [](const Keyed& a, const Keyed& c) { return a.cost > c.cost; });
+    std::vector<BatchItem> out(v.size());
+    for (size_t i = 0; i < v.size(); i++) out[i] = v[i].it;
+    if (evals) *evals = ev;
+    return out;
+}
+
+int batch_upload_items(disco_batch* b, const std::vector<BatchItem>& items)
+{
+    disco_ctx* ctx = b->ctx;
+    batch_release_graph(b);      // a captured graph holds d_items and nitems
+    if (items.size() > b->cap_items) {
+        if (b->d_items) DISCO_HIP_TRY(ctx, hipFree(b->d_items));
+        b->d_items = nullptr;
+        b->cap_items = 0;
+        DISCO_HIP_TRY(ctx, hipMalloc(&b->d_items, items.size() * sizeof(BatchItem)));
+        b->cap_items = items.size();
+    }
+    if (!items.empty())
+        DISCO_HIP_TRY(ctx, hipMemcpy(b->d_items, items.data(), items.size() * sizeof(BatchItem),
+                                     hipMemcpyHostToDevice));
+    b->nitems = (int)items.size();
+    return DISCO_OK;
+}
+
+template <int BLOCK>
+void batch_launch_block(const disco_batch* b, int groups, hipStream_t s, const double* x,
+                        const double* y, const double* z, double* ux, double* uy, double* uz,
+                        int accumulate)
+{
+    const LineLoop* lp = (const LineLoop*)b->d_loops;
+    const SegF* sf = (const SegF*)b->d_segf;
+    const SegD* sd = (const SegD*)b->d_segd;
+    const dim3 grid((unsigned)groups), block(BLOCK);
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, b->d_counters, 2);
+    if (b->prec == DISCO_PREC_FP64) {
+        hipLaunchKernelGGL((batch_line_kernel<BLOCK, PASS_FP64>), grid, block, 0, s, x, y, z, ux,
+                           uy, uz, b->d_samples, b->d_items, b->nitems, b->d_counters, lp, sf,
+                           sd, b->img, b->inner, accumulate);
+    } else {
+        hipLaunchKernelGGL((batch_line_kernel<BLOCK, PASS_FAR>), grid, block, 0, s, x, y, z, ux,
+                           uy, uz, b->d_samples, b->d_items, b->nitems, b->d_counters, lp, sf,
+                           sd, b->img, b->inner, accumulate);
+        hipLaunchKernelGGL((batch_line_kernel<BLOCK, PASS_NEAR>), grid, block, 0, s, x, y, z,
+                           ux, uy, uz, b->d_samples, b->d_items, b->nitems, b->d_counters + 1,
+                           lp, sf, sd, b->img, b->inner, accumulate);
+    }
+}
+
+int batch_launch(disco_batch* b, const disco_batch_tuning& t, hipStream_t s, const double* x,
+                 const double* y, const double* z, double* ux, double* uy, double* uz,
+                 int accumulate)
+{
+    if (b->nitems == 0) {
+        // nothing to sum: u = 0 (or unchanged) over every sample's atoms
+        if (!accumulate)
+            for (const BatchSample& S : b->samples)
+                if (S.n > 0)
+                    hipLaunchKernelGGL(zero3_kernel, dim3((S.n + kBlock - 1) / kBlock),
+                                       dim3(kBlock), 0, s, ux + S.a0, uy + S.a0, uz + S.a0, S.n);
+    } else if (t.block == 64) {
+        batch_launch_block<64>(b, t.groups, s, x, y, z, ux, uy, uz, accumulate);
+    } else if (t.block == 128) {
+        batch_launch_block<128>(b, t.groups, s, x, y, z, ux, uy, uz, accumulate);
+    } else {
+        batch_launch_block<256>(b, t.groups, s, x, y, z, ux, uy, uz, accumulate);
+    }
+    DISCO_HIP_TRY(b->ctx, hipGetLastError());
+    return DISCO_OK;
+}
+
+int default_groups(disco_ctx* ctx)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) return 1024;
+    return prop.multiProcessorCount * 4;
 }
 
 }  // namespace
@@ -652,16 +841,213 @@ int disco_hip_plan_kernel_ms(disco_plan* plan, double* ms, int nms)
     return DISCO_OK;
 }
 
+namespace {
+double shell_images(int img, int inner)
+{
+    const double o = 2.0 * img + 1.0;
+    const double i = inner >= 0 ? 2.0 * std::min(inner, img) + 1.0 : 0.0;
+    return o * o * o - i * i * i;
+}
+}  // namespace
+
 double disco_hip_plan_evals_per_atom(const disco_plan* plan)
 {
     if (!plan) return 0.0;
-    const double im = 2.0 * plan->img + 1.0;
-    return plan->work * im * im * im;
+    return plan->work * shell_images(plan->img, plan->inner);
+}
+
+int disco_hip_plan_set_inner(disco_plan* plan, int inner)
+{
+    if (!plan) return DISCO_ERR_ARG;
+    if (inner < -1 || inner > plan->img) return fail(plan->ctx, DISCO_ERR_ARG, "inner shell out of range");
+    plan->inner = inner;
+    return DISCO_OK;
+}
+
+int disco_hip_batch_set_inner(disco_batch* b, int inner)
+{
+    if (!b) return DISCO_ERR_ARG;
+    if (inner < -1 || inner > b->img) return fail(b->ctx, DISCO_ERR_ARG, "inner shell out of range");
+    if (inner != b->inner) {
+        b->inner = inner;
+        const double im = shell_images(b->img, inner);
+        b->evals = 0;
+        for (int k = 0; k < b->nsamples; k++) b->evals += b->samples[k].n * b->seg_per_sample[k] * im;
+        return batch_upload_items(b, make_items(b, b->tuning.tile, 1, nullptr));  // costs changed
+    }
+    return DISCO_OK;
 }
 
 int disco_hip_plan_free(disco_plan* plan)
 {
     free_plan(plan);
+    return DISCO_OK;
+}
+
+int disco_hip_batch_plan(disco_ctx* ctx, const disco_sample* samples, int nsamples, int img,
+                         int precision, disco_batch** out)
+{
+    if (!out) return fail(ctx, DISCO_ERR_ARG, "null batch pointer");
+    *out = nullptr;
+    int rc = check_common(ctx, 0, 1.0, 1.0, 1.0, img, precision, 0);
+    if (rc != DISCO_OK) return rc;
+    if (nsamples < 0 || (nsamples > 0 && !samples)) return fail(ctx, DISCO_ERR_ARG, "bad sample array");
+    disco_batch* b = new (std::nothrow) disco_batch();
+    if (!b) return fail(ctx, DISCO_ERR_OOM, "host allocation");
+    b->ctx = ctx; b->prec = precision; b->img = img; b->nsamples = nsamples;
+    LinePack all;
+    const double im = (2.0 * img + 1) * (2.0 * img + 1) * (2.0 * img + 1);
+    for (int k = 0; k < nsamples; k++) {
+        const disco_sample& S = samples[k];
+        std::string why;
+        if (S.n < 0 || S.atom_offset < 0) why = "negative atom range";
+        else if (S.nloops < 0 || (S.nloops > 0 && !S.loops)) why = "bad loop array";
+        else if (S.nloops > DISCO_MAX_LOOPS) { free_batch(b); return fail(ctx, DISCO_ERR_TOO_LARGE, "too many loops"); }
+        else if (!(std::isfinite(S.L[0]) && std::isfinite(S.L[1]) && std::isfinite(S.L[2]))) why = "non-finite box";
+        LinePack pk;
+        if (why.empty()) why = pack_line(S.loops, S.nloops, pk);
+        if (!why.empty()) {
+            free_batch(b);
+            return fail(ctx, DISCO_ERR_ARG, "sample " + std::to_string(k) + ": " + why);
+        }
+        BatchSample bs;
+        std::memset(&bs, 0, sizeof(bs));
+        bs.a0 = S.atom_offset; bs.n = S.n;
+        bs.l0 = (int)all.loops.size();
+        const int seg_base = (int)all.segd.size();
+        for (LineLoop L : pk.loops) { L.seg0 += seg_base; all.loops.push_back(L); }
+        all.segd.insert(all.segd.end(), pk.segd.begin(), pk.segd.end());
+        all.segf.insert(all.segf.end(), pk.segf.begin(), pk.segf.end());
+        bs.l1 = (int)all.loops.size();
+        bs.Lx = S.L[0]; bs.Ly = S.L[1]; bs.Lz = S.L[2];
+        b->samples.push_back(bs);
+        b->seg_per_sample.push_back(pk.segments);
+        b->evals += (double)S.n * pk.segments * im;
+    }
+    disco_ctx* c = ctx;
+    auto up = [&](void** dst, const void* src, size_t bytes) -> int {
+        if (!bytes) return DISCO_OK;
+        DISCO_HIP_TRY(c, hipMalloc(dst, bytes));
+        DISCO_HIP_TRY(c, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        return DISCO_OK;
+    };
+    if ((rc = hipSetDevice(ctx->device) == hipSuccess ? DISCO_OK : DISCO_ERR_HIP) != DISCO_OK ||
+        (rc = up(&b->d_loops, all.loops.data(), all.loops.size() * sizeof(LineLoop))) != DISCO_OK ||
+        (rc = up(&b->d_segf, all.segf.data(), all.segf.size() * sizeof(SegF))) != DISCO_OK ||
+        (rc = up(&b->d_segd, all.segd.data(), all.segd.size() * sizeof(SegD))) != DISCO_OK ||
+        (rc = up((void**)&b->d_samples, b->samples.data(), b->samples.size() * sizeof(BatchSample))) != DISCO_OK) {
+        free_batch(b);
+        return rc;
+    }
+    if (hipMalloc(&b->d_counters, 64 * sizeof(unsigned)) != hipSuccess ||
+        hipStreamCreate(&b->cap_stream) != hipSuccess) {
+        free_batch(b);
+        return fail(ctx, DISCO_ERR_HIP, "batch buffers");
+    }
+    b->tuning = {256, 256, default_groups(ctx)};
+    rc = batch_upload_items(b, make_items(b, b->tuning.tile, 1, nullptr));
+    if (rc != DISCO_OK) { free_batch(b); return rc; }
+    *out = b;
+    return DISCO_OK;
+}
+
+int disco_hip_batch_set_tuning(disco_batch* b, const disco_batch_tuning* t)
+{
+    if (!b || !t) return DISCO_ERR_ARG;
+    if (!tuning_valid(*t)) return fail(b->ctx, DISCO_ERR_ARG, "invalid launch shape");
+    if (t->tile != b->tuning.tile) {
+        const int rc = batch_upload_items(b, make_items(b, t->tile, 1, nullptr));
+        if (rc != DISCO_OK) return rc;
+    }
+    b->tuning = *t;
+    return DISCO_OK;
+}
+
+int disco_hip_batch_get_tuning(const disco_batch* b, disco_batch_tuning* t)
+{
+    if (!b || !t) return DISCO_ERR_ARG;
+    *t = b->tuning;
+    return DISCO_OK;
+}
+
+int disco_hip_batch_run(disco_batch* b, const double* x, const double* y, const double* z,
+                        double* ux, double* uy, double* uz, int accumulate, void* stream)
+{
+    if (!b) return DISCO_ERR_ARG;
+    disco_ctx* ctx = b->ctx;
+    if (b->nsamples > 0 && (!x || !y || !z || !ux || !uy || !uz))
+        return fail(ctx, DISCO_ERR_ARG, "null device array");
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const void* key[7] = {x, y, z, ux, uy, uz, (const void*)(intptr_t)(accumulate ? 1 : 0)};
+    const bool same = b->exec && std::memcmp(key, b->key, sizeof(key)) == 0 &&
+                      std::memcmp(&b->tuning, &b->key_tuning, sizeof(b->tuning)) == 0;
+    if (!same) {
+        batch_release_graph(b);
+        DISCO_HIP_TRY(ctx, hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_launch(b, b->tuning, b->cap_stream, x, y, z, ux, uy, uz, accumulate);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+        if (rc != DISCO_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return fail(ctx, DISCO_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        b->graph = g;
+        DISCO_HIP_TRY(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        std::memcpy(b->key, key, sizeof(key));
+        b->key_tuning = b->tuning;
+    }
+    DISCO_HIP_TRY(ctx, hipGraphLaunch(b->exec, (hipStream_t)stream));
+    return DISCO_OK;
+}
+
+int disco_hip_batch_autotune(disco_batch* b, const double* x, const double* y, const double* z,
+                             double* ux, double* uy, double* uz, int stride, void* stream,
+                             disco_batch_tuning* best, double* best_ms)
+{
+    if (!b || stride < 1) return DISCO_ERR_ARG;
+    disco_ctx* ctx = b->ctx;
+    if (b->nsamples > 0 && (!x || !y || !z || !ux || !uy || !uz))
+        return fail(ctx, DISCO_ERR_ARG, "null device array");
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    const int cus = default_groups(ctx) / 4;
+    std::vector<disco_batch_tuning> cand;
+    for (int block : {128, 256})
+        for (int mult : {1, 4})
+            for (int g : {4, 8, 16})
+                cand.push_back({block, block * mult, cus * g});
+    const disco_batch_tuning saved = b->tuning;
+    disco_batch_tuning win = saved;
+    double win_ms = 1e300, full_evals = b->evals;
+    for (const disco_batch_tuning& t : cand) {
+        double ev = 0;
+        int rc = batch_upload_items(b, make_items(b, t.tile, stride, &ev));
+        if (rc != DISCO_OK) return rc;
+        if (b->nitems == 0) continue;
+        rc = batch_launch(b, t, s, x, y, z, ux, uy, uz, 0);        // warm
+        if (rc != DISCO_OK) return rc;
+        DISCO_HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+        rc = batch_launch(b, t, s, x, y, z, ux, uy, uz, 0);
+        if (rc != DISCO_OK) return rc;
+        DISCO_HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+        DISCO_HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+        float ms = 0.f;
+        DISCO_HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        const double scaled = ev > 0 ? ms * full_evals / ev : ms;
+        if (scaled < win_ms) { win_ms = scaled; win = t; }
+    }
+    const int rc = batch_upload_items(b, make_items(b, win.tile, 1, nullptr));
+    if (rc != DISCO_OK) return rc;
+    b->tuning = win;
+    if (best) *best = win;
+    if (best_ms) *best_ms = win_ms < 1e300 ? win_ms : 0.0;
+    return DISCO_OK;
+}
+
+double disco_hip_batch_evals(const disco_batch* b) { return b ? b->evals : 0.0; }
+
+int disco_hip_batch_free(disco_batch* b)
+{
+    if (b && b->ctx) (void)hipSetDevice(b->ctx->device);
+    free_batch(b);
     return DISCO_OK;
 }
 

@@ -57,8 +57,8 @@ struct alignas(16 * sizeof(R)) Seg {
     R tx, ty, tz;     // unit tangent of A -> B
     R L;              // |B - A|
     R nx, ny, nz;     // (A - c) x (B - c)
-    R wx, wy, wz;     // b x t
-    R pad0, pad1, pad2;
+    R wkx, wky, wkz;  // cK (b x t)
+    R wjx, wjy, wjz;  // cJ (b x t)
 };
 using SegF = Seg<float>;
 using SegD = Seg<double>;
@@ -119,7 +119,6 @@ template <class M>
 DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restrict__ s, int nv,
                              float bx, float by, float bz, float& ox, float& oy, float& oz)
 {
-    const float cJ = (float)kCJ, cK = (float)kC3;
     const float l1sq = qx * qx + qy * qy + qz * qz;
     const float rl1 = M::rsqrt_f(l1sq);
     const float l1 = l1sq * rl1;
@@ -136,6 +135,7 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
     float swx = 0.f, swy = 0.f, swz = 0.f;
     float sax = 0.f, say = 0.f, saz = 0.f;
     float stx = 0.f, sty = 0.f, stz = 0.f;
+#pragma unroll 2
     for (int k = 0; k < nv; k++) {
         const SegF& g = s[k];
         const float px = g.bx - qx, py = g.by - qy, pz = g.bz - qz;       // B - p
@@ -162,24 +162,27 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
             Y = M::ldexp_f(Y, -e);
         }
 
-        // segment terms
+        // segment terms; K0 = kn / kd, one reciprocal shared with J0's argument
         const float sA = ax * g.tx + ay * g.ty + az * g.tz;
         const float sB = sA + g.L;
-        const float J0 = two_atanh_f<M>(g.L * M::rcp_f(RA + RB));
+        const float RS = RA + RB;
+        float kn = g.L * (sA + sB) * (rA * rB);
+        float kd = sB * RA + sA * RB;
+        if ((sA < 0.0f) && (sB > 0.0f)) {          // atom beside the segment
+            const float dx = ax - sA * g.tx, dy = ay - sA * g.ty, dz = az - sA * g.tz;
+            kd = dx * dx + dy * dy + dz * dz;
+            kn = sB * rB - sA * rA;
+        }
+        const float rr = M::rcp_f(RS * kd);
+        const float J0 = two_atanh_f<M>(g.L * kd * rr);
+        const float K0 = kn * RS * rr;
         const float K1 = rA - rB;
-        const bool mixed = (sA < 0.0f) && (sB > 0.0f);
-        const float dx = ax - sA * g.tx, dy = ay - sA * g.ty, dz = az - sA * g.tz;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        const float kn = mixed ? (sB * rB - sA * rA) : g.L * (sA + sB) * (rA * rB);
-        const float kd = mixed ? d2 : (sB * RA + sA * RB);
-        const float K0 = kn * M::rcp_f(kd);
-        const float cdw = cK * (ax * g.wx + ay * g.wy + az * g.wz);
-        const float al = cdw * K0;
-        const float be = cdw * (K1 - sA * K0);
-        const float jj = cJ * J0;
-        swx += jj * g.wx; swy += jj * g.wy; swz += jj * g.wz;
-        sax += al * ax;   say += al * ay;   saz += al * az;
-        stx += be * g.tx; sty += be * g.ty; stz += be * g.tz;
+        const float dw = ax * g.wkx + ay * g.wky + az * g.wkz;   // cK (d.w) = cK (a.w)
+        const float al = dw * K0;
+        const float be = dw * (K1 - sA * K0);
+        swx += J0 * g.wjx; swy += J0 * g.wjy; swz += J0 * g.wjz;
+        sax += al * ax;    say += al * ay;    saz += al * az;
+        stx += be * g.tx;  sty += be * g.ty;  stz += be * g.tz;
 
         ax = px; ay = py; az = pz;
         RA = RB; rA = rB;
@@ -195,7 +198,6 @@ template <class M>
 DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __restrict__ s, int nv,
                              double bx, double by, double bz, double& ox, double& oy, double& oz)
 {
-    const double cJ = kCJ, cK = kC3;
     const double l1sq = qx * qx + qy * qy + qz * qz;
     const double l1 = M::sqrt_d(l1sq);
     const double rl1 = 1.0 / l1;
@@ -233,23 +235,24 @@ DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __rest
 
         const double sA = ax * g.tx + ay * g.ty + az * g.tz;
         const double sB = sA + g.L;
-        const double J0 = two_atanh_d<M>(g.L / (RA + RB));
-        const double K1 = rA - rB;
-        const bool mixed = (sA < 0.0) && (sB > 0.0);
-        double K0;
-        if (mixed) {
+        const double RS = RA + RB;
+        double kn = g.L * (sA + sB) * (rA * rB);
+        double kd = sB * RA + sA * RB;
+        if ((sA < 0.0) && (sB > 0.0)) {
             const double dx = ax - sA * g.tx, dy = ay - sA * g.ty, dz = az - sA * g.tz;
-            K0 = (sB * rB - sA * rA) / (dx * dx + dy * dy + dz * dz);
-        } else {
-            K0 = g.L * (sA + sB) * (rA * rB) / (sB * RA + sA * RB);
+            kd = dx * dx + dy * dy + dz * dz;
+            kn = sB * rB - sA * rA;
         }
-        const double cdw = cK * (ax * g.wx + ay * g.wy + az * g.wz);
-        const double al = cdw * K0;
-        const double be = cdw * (K1 - sA * K0);
-        const double jj = cJ * J0;
-        swx += jj * g.wx; swy += jj * g.wy; swz += jj * g.wz;
-        sax += al * ax;   say += al * ay;   saz += al * az;
-        stx += be * g.tx; sty += be * g.ty; stz += be * g.tz;
+        const double rr = 1.0 / (RS * kd);
+        const double J0 = two_atanh_d<M>(g.L * kd * rr);
+        const double K0 = kn * RS * rr;
+        const double K1 = rA - rB;
+        const double dw = ax * g.wkx + ay * g.wky + az * g.wkz;
+        const double al = dw * K0;
+        const double be = dw * (K1 - sA * K0);
+        swx += J0 * g.wjx; swy += J0 * g.wjy; swz += J0 * g.wjz;
+        sax += al * ax;    say += al * ay;    saz += al * az;
+        stx += be * g.tx;  sty += be * g.ty;  stz += be * g.tz;
 
         ax = px; ay = py; az = pz;
         RA = RB; rA = rB;

@@ -110,16 +110,19 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, LinePack& ou
             d.nx = ay[j] * az[k] - az[j] * ay[k];
             d.ny = az[j] * ax[k] - ax[j] * az[k];
             d.nz = ax[j] * ay[k] - ay[j] * ax[k];
-            d.wx = lp.b[1] * d.tz - lp.b[2] * d.ty;
-            d.wy = lp.b[2] * d.tx - lp.b[0] * d.tz;
-            d.wz = lp.b[0] * d.ty - lp.b[1] * d.tx;
+            const double wx = lp.b[1] * d.tz - lp.b[2] * d.ty;
+            const double wy = lp.b[2] * d.tx - lp.b[0] * d.tz;
+            const double wz = lp.b[0] * d.ty - lp.b[1] * d.tx;
+            d.wkx = kC3 * wx; d.wky = kC3 * wy; d.wkz = kC3 * wz;
+            d.wjx = kCJ * wx; d.wjy = kCJ * wy; d.wjz = kCJ * wz;
             SegF f;
             std::memset(&f, 0, sizeof(f));
             f.bx = (float)d.bx; f.by = (float)d.by; f.bz = (float)d.bz;
             f.tx = (float)d.tx; f.ty = (float)d.ty; f.tz = (float)d.tz;
             f.L = (float)d.L;
             f.nx = (float)d.nx; f.ny = (float)d.ny; f.nz = (float)d.nz;
-            f.wx = (float)d.wx; f.wy = (float)d.wy; f.wz = (float)d.wz;
+            f.wkx = (float)d.wkx; f.wky = (float)d.wky; f.wkz = (float)d.wkz;
+            f.wjx = (float)d.wjx; f.wjy = (float)d.wjy; f.wjz = (float)d.wjz;
             out.segd.push_back(d);
             out.segf.push_back(f);
         }

@@ -1,0 +1,329 @@
+"""Learned far image shell of the periodic sum (host side, on the C ABI).
+
+The field of the loops over images (i, j, k) with max(|i|,|j|,|k|) in (1, img]
+is the far shell.  Its surrogate is
+
+    far(p) = sum_loops [ dipole_far(p) + scale * MLP(features(p, loop)) ]
+
+* ``dipole_far`` -- the loop replaced by one loopon of its area at its centre
+  (the Kroupa term, Code/kroupa.hpp:52-73), summed exactly over the far shell
+  by the loopon kernels (``Plan`` with ``inner = 1``);
+* ``MLP`` -- 14 -> 128 -> 128 -> 128 -> 3 (SiLU), trained by the library's own
+  GPU kernels (``disco_hip_surrogate_*``) on exact far shells generated by the
+  line kernels, with ``scale = |b| (R / L)^4``.
+
+The near shell (+-1) is always the exact line integral.  ``far_shell`` checks
+the surrogate against the exact far shell on a validation subset of atoms and
+falls back to the exact sum when the error exceeds the tolerance -- the plain
+sum is both the fallback and the judge.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import backend as B
+from . import geometry as G
+
+SHAPES = {"circle": 0, "rect": 1, "parallelogram": 2, "polygon": 3}
+NIN, NOUT = 14, 3
+
+
+class _SLoop(ctypes.Structure):
+    _fields_ = [("c", ctypes.c_double * 3), ("n", ctypes.c_double * 3), ("b", ctypes.c_double * 3),
+                ("radius", ctypes.c_double), ("shape", ctypes.c_int)]
+
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_bound = False
+
+
+def _lib():
+    global _bound
+    L = B.lib()
+    if not _bound:
+        L.disco_hip_surrogate_num_params.restype = ctypes.c_int
+        L.disco_hip_surrogate_features.argtypes = [_dp, ctypes.c_int, ctypes.POINTER(_SLoop), _dp, _fp]
+        L.disco_hip_surrogate_scale.argtypes = [ctypes.POINTER(_SLoop), _dp]
+        L.disco_hip_surrogate_scale.restype = ctypes.c_double
+        L.disco_hip_surrogate_create.argtypes = [_vp, ctypes.c_uint, ctypes.POINTER(_vp)]
+        L.disco_hip_surrogate_free.argtypes = [_vp]
+        L.disco_hip_surrogate_get_params.argtypes = [_vp, _fp]
+        L.disco_hip_surrogate_set_params.argtypes = [_vp, _fp]
+        L.disco_hip_surrogate_forward.argtypes = [_vp, _fp, ctypes.c_int, _fp]
+        L.disco_hip_surrogate_loss_grad.argtypes = [_vp, _fp, _fp, ctypes.c_int, _fp, _fp]
+        L.disco_hip_surrogate_train.argtypes = [_vp, _fp, _fp, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_float, ctypes.c_uint, _fp]
+        L.disco_hip_surrogate_apply.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                ctypes.POINTER(_SLoop), ctypes.c_int, _dp,
+                                                ctypes.c_int, _vp]
+        L.disco_hip_plan_set_inner.argtypes = [_vp, ctypes.c_int]
+        L.disco_hip_batch_set_inner.argtypes = [_vp, ctypes.c_int]
+        _bound = True
+    return L
+
+
+def num_params() -> int:
+    return _lib().disco_hip_surrogate_num_params()
+
+
+# ------------------------------------------------------------------ loop descriptions
+@dataclass
+class LoopShapeInfo:
+    """What the surrogate knows of a loop: centre, normal, b, radius, shape."""
+    c: tuple
+    n: tuple
+    b: tuple
+    radius: float
+    shape: str
+
+    def struct(self):
+        s = _SLoop()
+        s.c[:] = list(self.c)
+        s.n[:] = list(self.n)
+        s.b[:] = list(self.b)
+        s.radius = float(self.radius)
+        s.shape = SHAPES[self.shape]
+        return s
+
+
+def shape_info(lp: B.LoopData, radius: float, shape: str) -> LoopShapeInfo:
+    """Surrogate description of a line loop (normal from its vector area)."""
+    V = np.stack([lp.vx, lp.vy, lp.vz], 1) - np.array([lp.cx, lp.cy, lp.cz])
+    N = np.cross(np.roll(V, 1, axis=0), V).sum(0)
+    n = N / np.linalg.norm(N)
+    return LoopShapeInfo((lp.cx, lp.cy, lp.cz), tuple(n), tuple(lp.burgers()), radius, shape)
+
+
+def dipole_loopon(lp: B.LoopData) -> B.LoopData:
+    """The loop as one loopon of its area at its centre (same b, normal from
+    the polyline's vector area, m = b / |b|)."""
+    V = np.stack([lp.vx, lp.vy, lp.vz], 1) - np.array([lp.cx, lp.cy, lp.cz])
+    N = np.cross(np.roll(V, 1, axis=0), V).sum(0)
+    area = 0.5 * np.linalg.norm(N)
+    n = N / np.linalg.norm(N)
+    b = np.array(lp.burgers())
+    bm = float(np.linalg.norm(b))
+    m = b / bm
+    return B.LoopData(phys=(bm, bm, bm, *m, *n, area), line=False, x=np.array([lp.cx]),
+                      y=np.array([lp.cy]), z=np.array([lp.cz]))
+
+
+def features(points, info: LoopShapeInfo, box) -> np.ndarray:
+    """Network inputs of atoms `points` (n x 3) for one loop (n x 14 float32)."""
+    p = np.ascontiguousarray(points, np.float64)
+    out = np.zeros((p.shape[0], NIN), np.float32)
+    L = (ctypes.c_double * 3)(*[float(t) for t in box])
+    s = info.struct()
+    rc = _lib().disco_hip_surrogate_features(p.ctypes.data_as(_dp), p.shape[0], ctypes.byref(s),
+                                             L, out.ctypes.data_as(_fp))
+    if rc:
+        raise B.DiscoError(rc, "features")
+    return out
+
+
+def scale(info: LoopShapeInfo, box) -> float:
+    L = (ctypes.c_double * 3)(*[float(t) for t in box])
+    s = info.struct()
+    return _lib().disco_hip_surrogate_scale(ctypes.byref(s), L)
+
+
+# ------------------------------------------------------------------ the network
+class Surrogate:
+    """The MLP's parameters and optimizer state on the device."""
+
+    def __init__(self, ctx: B.Context, seed: int = 1):
+        self.ctx = ctx
+        self._h = _vp()
+        ctx.check(_lib().disco_hip_surrogate_create(ctx.handle, seed, ctypes.byref(self._h)))
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(num_params(), np.float32)
+        self.ctx.check(_lib().disco_hip_surrogate_get_params(self._h, out.ctypes.data_as(_fp)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == num_params()
+        self.ctx.check(_lib().disco_hip_surrogate_set_params(self._h, p.ctypes.data_as(_fp)))
+
+    def forward(self, feats) -> np.ndarray:
+        f = np.ascontiguousarray(feats, np.float32)
+        out = np.zeros((f.shape[0], NOUT), np.float32)
+        self.ctx.check(_lib().disco_hip_surrogate_forward(self._h, f.ctypes.data_as(_fp),
+                                                          f.shape[0], out.ctypes.data_as(_fp)))
+        return out
+
+    def loss_grad(self, feats, targets):
+        f = np.ascontiguousarray(feats, np.float32)
+        t = np.ascontiguousarray(targets, np.float32)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(num_params(), np.float32)
+        self.ctx.check(_lib().disco_hip_surrogate_loss_grad(self._h, f.ctypes.data_as(_fp),
+                                                            t.ctypes.data_as(_fp), f.shape[0],
+                                                            loss.ctypes.data_as(_fp),
+                                                            g.ctypes.data_as(_fp)))
+        return float(loss[0]), g
+
+    def train(self, feats, targets, epochs=20, batch=4096, lr=2e-3, seed=7) -> float:
+        f = np.ascontiguousarray(feats, np.float32)
+        t = np.ascontiguousarray(targets, np.float32)
+        loss = np.zeros(1, np.float32)
+        self.ctx.check(_lib().disco_hip_surrogate_train(self._h, f.ctypes.data_as(_fp),
+                                                        t.ctypes.data_as(_fp), f.shape[0],
+                                                        epochs, batch, lr, seed,
+                                                        loss.ctypes.data_as(_fp)))
+        return float(loss[0])
+
+    def apply_torch(self, x, y, z, ux, uy, uz, infos, box, chunk_rows=1 << 22):
+        """u += sum over loops of scale * MLP (device atoms as torch tensors)."""
+        import torch
+        arr = (_SLoop * max(1, len(infos)))(*[i.struct() for i in infos])
+        L = (ctypes.c_double * 3)(*[float(t) for t in box])
+        stream = torch.cuda.current_stream().cuda_stream
+        self.ctx.check(_lib().disco_hip_surrogate_apply(
+            self._h, x.numel(), x.data_ptr(), y.data_ptr(), z.data_ptr(), ux.data_ptr(),
+            uy.data_ptr(), uz.data_ptr(), arr, len(infos), L, chunk_rows, stream))
+
+    def free(self):
+        if self._h:
+            _lib().disco_hip_surrogate_free(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ exact far shells
+def set_inner(plan: B.Plan, inner: int):
+    plan.ctx.check(_lib().disco_hip_plan_set_inner(plan._h, int(inner)))
+
+
+def exact_far_shell_torch(ctx, x, y, z, loops, box, img, prec=B.PREC_FP32C):
+    """Exact far shell (images with max index in (1, img]) of line loops."""
+    import torch
+    u = [torch.zeros_like(x) for _ in range(3)]
+    plan = B.Plan(ctx, loops, box, img, prec, kind="line")
+    set_inner(plan, 1)
+    plan.run_torch(x, y, z, *u)
+    plan.free()
+    return u
+
+
+def dipole_far_shell_torch(ctx, x, y, z, loops, box, img, u=None, accumulate=False):
+    """Far shell of the loops' dipole terms, FP64 (exact for that term)."""
+    import torch
+    if u is None:
+        u = [torch.zeros_like(x) for _ in range(3)]
+    plan = B.Plan(ctx, [dipole_loopon(l) for l in loops], box, img, B.PREC_FP64, kind="loopon")
+    set_inner(plan, 1)
+    plan.run_torch(x, y, z, *u, accumulate=accumulate)
+    plan.free()
+    return u
+
+
+# ------------------------------------------------------------------ training data
+def random_fcc_loops(rng, nloops, a0=3.615, cells=100):
+    """Loops drawn as in the 4M-atom workload: radius U(10, 60), centre U(box)
+    snapped, n one of the 4 {111}, b one of the 3 in-plane a/2<110>; 60 %
+    circles of 48 segments, 40 % hexagons.  Returns (LoopData, info) pairs."""
+    loops, geo = G._fcc_loop_set(rng, nloops, a0, cells)
+    out = []
+    for lp, (kind, R, c, n, b) in zip(loops, geo):
+        info = shape_info(lp, R, "circle" if kind == 0 else "polygon")
+        out.append((lp, info))
+    return out
+
+
+def make_training_set(ctx, nloops=2000, atoms_per_loop=500, img=4, seed=11, a0=3.615, cells=100):
+    """Exact far shells of random (atom, loop) pairs, minus the dipole far shell,
+    divided by each loop's scale: (features n x 14, targets n x 3, scales n).
+    Computed on the GPU: each loop is one sample of a batched launch."""
+    import torch
+    rng = np.random.default_rng(seed)
+    L = a0 * cells
+    box = (L, L, L)
+    pairs = random_fcc_loops(rng, nloops, a0, cells)
+    samples, feats, scales, pts = [], [], [], []
+    for lp, info in pairs:
+        p = rng.uniform(-L / 2, L / 2, size=(atoms_per_loop, 3))
+        pts.append(p)
+        samples.append(G.Sample("train", p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), [lp],
+                                box, img))
+        feats.append(features(p, info, box))
+        scales.append(np.full(atoms_per_loop, scale(info, box)))
+    P = np.concatenate(pts)
+    dev = torch.device("cuda")
+    x, y, z = (torch.from_numpy(np.ascontiguousarray(P[:, k])).to(dev) for k in range(3))
+    u = [torch.zeros_like(x) for _ in range(3)]
+    bt = B.Batch(ctx, samples, img, B.PREC_FP64)
+    ctx.check(_lib().disco_hip_batch_set_inner(bt._h, 1))
+    bt.run_torch(x, y, z, *u)
+    torch.cuda.synchronize()
+    exact = np.stack([t.cpu().numpy() for t in u], 1)
+    bt.free()
+    # dipole far shell, loop by loop on its own atoms
+    dip = np.zeros_like(exact)
+    for k, (lp, _info) in enumerate(pairs):
+        s = slice(k * atoms_per_loop, (k + 1) * atoms_per_loop)
+        ud = dipole_far_shell_torch(ctx, x[s].contiguous(), y[s].contiguous(),
+                                    z[s].contiguous(), [lp], box, img)
+        dip[s] = np.stack([t.cpu().numpy() for t in ud], 1)
+    S = np.concatenate(scales)
+    targets = ((exact - dip) / S[:, None]).astype(np.float32)
+    return np.concatenate(feats), targets, S, exact, dip
+
+
+# ------------------------------------------------------------------ the far-shell pipeline
+@dataclass
+class FarShellReport:
+    used_surrogate: bool
+    max_err: float          # surrogate vs exact far shell on the validation atoms (A)
+    dipole_only_err: float  # the same without the network (A)
+    n_valid: int
+    seconds_surrogate: float
+    seconds_exact: float | None
+
+
+def far_shell(ctx, sur: Surrogate, x, y, z, u, loops, infos, box, img, tol=1e-5,
+              n_valid=4096, seed=3, exact_if_failed=True):
+    """u += far shell of the loops at the device atoms (x, y, z): the surrogate
+    (dipole + MLP) when its error on n_valid random atoms is within tol of the
+    exact far shell, else the exact far shell.  Returns a FarShellReport."""
+    import torch
+    n = x.numel()
+    rng = np.random.default_rng(seed)
+    idx = torch.from_numpy(np.sort(rng.choice(n, min(n_valid, n), replace=False))).to(x.device)
+    xv, yv, zv = x[idx].contiguous(), y[idx].contiguous(), z[idx].contiguous()
+    # judge: exact far shell on the validation atoms
+    ex = exact_far_shell_torch(ctx, xv, yv, zv, loops, box, img)
+    dv = dipole_far_shell_torch(ctx, xv, yv, zv, loops, box, img)
+    dip_only = max(float((e - d).abs().max()) for e, d in zip(ex, dv))
+    sur.apply_torch(xv, yv, zv, *dv, infos, box)
+    torch.cuda.synchronize()
+    err = max(float((e - d).abs().max()) for e, d in zip(ex, dv))
+    if err <= tol:
+        t0 = time.perf_counter()
+        dipole_far_shell_torch(ctx, x, y, z, loops, box, img, u=u, accumulate=True)
+        sur.apply_torch(x, y, z, *u, infos, box)
+        torch.cuda.synchronize()
+        return FarShellReport(True, err, dip_only, idx.numel(), time.perf_counter() - t0, None)
+    t_ex = None
+    if exact_if_failed:
+        t0 = time.perf_counter()
+        plan = B.Plan(ctx, loops, box, img, B.PREC_FP32C, kind="line")
+        set_inner(plan, 1)
+        plan.run_torch(x, y, z, *u, accumulate=True)
+        torch.cuda.synchronize()
+        plan.free()
+        t_ex = time.perf_counter() - t0
+    return FarShellReport(False, err, dip_only, idx.numel(), 0.0, t_ex)

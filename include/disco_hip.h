@@ -145,9 +145,110 @@ int disco_hip_plan_profile(disco_plan *plan, int on);
  * (FP64 sum, FP32 far pass, or loopon sum), ms[1] = the FP64 near pass of the
  * FP32C line mode (0 otherwise).  nms = capacity of ms (>= 2). */
 int disco_hip_plan_kernel_ms(disco_plan *plan, double *ms, int nms);
+/* Sum only the image shell: images (i, j, k) with max(|i|, |j|, |k|) <= inner
+ * are skipped (inner = -1, the default, keeps them all; inner <= img).  A
+ * far-shell plan (inner = 1) is what the surrogate replaces and validates. */
+int disco_hip_plan_set_inner(disco_plan *plan, int inner);
 /* Work of one run per atom: segments (or loopons) x images. */
 double disco_hip_plan_evals_per_atom(const disco_plan *plan);
 int disco_hip_plan_free(disco_plan *plan);
+
+/* ---------------------------------------------------------------- batches
+ * Many independent samples -- each with its own atoms, loops and box -- summed
+ * by ONE persistent launch per pass, captured in a HIP graph on first run and
+ * replayed after.  Workgroups take (sample, atom tile) items from a queue;
+ * items are ordered longest first.  Replaces calling kroupaGPU once per sample
+ * (Code/backend_cuda.cu:568-722) for batched sample generation. */
+typedef struct {
+    long long atom_offset;    /* first atom of the sample in the concatenated arrays */
+    int n;                    /* its atom count                                       */
+    int nloops;
+    const disco_line_loop *loops;
+    double L[3];              /* its periodic box                                     */
+} disco_sample;
+
+/* Launch shape of the persistent kernels. */
+typedef struct {
+    int block;                /* workgroup size: 64, 128 or 256                      */
+    int tile;                 /* atoms per work item, a multiple of block            */
+    int groups;               /* persistent workgroups per launch (>= 1)             */
+} disco_batch_tuning;
+
+typedef struct disco_batch disco_batch;
+
+int disco_hip_batch_plan(disco_ctx *ctx, const disco_sample *samples, int nsamples, int img,
+                         int precision, disco_batch **batch);
+int disco_hip_batch_set_tuning(disco_batch *batch, const disco_batch_tuning *tuning);
+int disco_hip_batch_get_tuning(const disco_batch *batch, disco_batch_tuning *tuning);
+/* Times each candidate launch shape on every `stride`-th sample (stride >= 1)
+ * and keeps the fastest; writes it to best (may be NULL) and its time, scaled
+ * to the whole batch, to best_ms (may be NULL).  Synchronous; it overwrites u. */
+int disco_hip_batch_autotune(disco_batch *batch, const double *d_x, const double *d_y,
+                             const double *d_z, double *d_ux, double *d_uy, double *d_uz,
+                             int stride, void *stream, disco_batch_tuning *best,
+                             double *best_ms);
+/* One graph launch over all samples (asynchronous on `stream`). */
+int disco_hip_batch_run(disco_batch *batch, const double *d_x, const double *d_y,
+                        const double *d_z, double *d_ux, double *d_uy, double *d_uz,
+                        int accumulate, void *stream);
+/* Image shell of a batch, as disco_hip_plan_set_inner. */
+int disco_hip_batch_set_inner(disco_batch *batch, int inner);
+/* atom x segment x image evaluations of one run. */
+double disco_hip_batch_evals(const disco_batch *batch);
+int disco_hip_batch_free(disco_batch *batch);
+
+/* ---------------------------------------------------------------- surrogate
+ * The learned far image shell.  The field of a loop over the images with
+ * max(|i|,|j|,|k|) in (1, img] is the far shell; it is the far-shell sum of
+ * the loop's dipole term (one loopon of the loop's area at its centre, summed
+ * exactly by a loopon plan with disco_hip_plan_set_inner) plus a residual that
+ * an MLP predicts: 14 features -> 128 -> 128 -> 128 -> 3 (SiLU), layer 1 in
+ * FP32, layers 2-4 BF16 on MFMA with FP32 accumulation, output scaled per loop
+ * by disco_hip_surrogate_scale.  It is trained (Adam, FP32 masters) by the
+ * library's own forward and backward kernels.  No counterpart in the
+ * reference, which sums +-1 images only (Code/backend_cpu.cpp:51-53). */
+#define DISCO_SURR_IN 14
+#define DISCO_SURR_OUT 3
+
+typedef struct {
+    double c[3];              /* loop centre                                          */
+    double n[3];              /* unit plane normal                                    */
+    double b[3];              /* Burgers vector                                       */
+    double radius;            /* radius (circumradius of a polygon), Angstrom         */
+    int shape;                /* LoopShape (Code/loop.hpp:13): 0 circle, 1 rectangle,
+                                 2 parallelogram, 3 polygon                           */
+} disco_surrogate_loop;
+
+typedef struct disco_surrogate disco_surrogate;
+
+int disco_hip_surrogate_num_params(void);
+/* Features of atoms p (n x 3, host) for one loop in box L (3): n x 14 floats. */
+int disco_hip_surrogate_features(const double *p, int n, const disco_surrogate_loop *loop,
+                                 const double *L, float *out);
+/* Output scale of a loop: |b| (radius / mean(L))^4. */
+double disco_hip_surrogate_scale(const disco_surrogate_loop *loop, const double *L);
+int disco_hip_surrogate_create(disco_ctx *ctx, unsigned seed, disco_surrogate **sur);
+int disco_hip_surrogate_free(disco_surrogate *sur);
+/* FP32 master parameters (disco_hip_surrogate_num_params floats). */
+int disco_hip_surrogate_get_params(disco_surrogate *sur, float *out);
+int disco_hip_surrogate_set_params(disco_surrogate *sur, const float *in);
+/* Network output for n host feature rows (n x 14) into out (n x 3). */
+int disco_hip_surrogate_forward(disco_surrogate *sur, const float *feats, int n, float *out);
+/* Mean squared error over n rows and its gradient (all parameters). */
+int disco_hip_surrogate_loss_grad(disco_surrogate *sur, const float *feats,
+                                  const float *targets, int n, float *loss, float *grads);
+/* Adam over a host data set (features n x 14, targets n x 3), cosine-decayed
+ * learning rate from lr to lr / 100; final_loss = last batch's loss. */
+int disco_hip_surrogate_train(disco_surrogate *sur, const float *feats, const float *targets,
+                              int n, int epochs, int batch, float lr, unsigned seed,
+                              float *final_loss);
+/* u += sum over loops of scale * network(features) for device atoms; rows of
+ * (atom, loop) pairs are processed chunk_rows at a time (<= 0: 4M).
+ * Synchronous on `stream`. */
+int disco_hip_surrogate_apply(disco_surrogate *sur, int natoms, const double *d_x,
+                              const double *d_y, const double *d_z, double *d_ux, double *d_uy,
+                              double *d_uz, const disco_surrogate_loop *loops, int nloops,
+                              const double *L, int chunk_rows, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,113 @@
+"""Batched samples (one persistent launch per pass, HIP-graph replay) against
+one-sample plans and the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from disco_amd import backend as B
+from disco_amd import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_batch(nsamples=10, seed=7):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nsamples):
+        s = G.config_c4_sample(rng)
+        keep = np.arange(0, s.n, 7)            # smaller atom sets, uneven sizes
+        out.append(s.subsample(keep))
+    return out
+
+
+def _concat(samples):
+    import torch
+    cat = lambda k: torch.from_numpy(np.concatenate([getattr(s, k) for s in samples])).cuda()
+    return cat("x"), cat("y"), cat("z")
+
+
+@pytest.mark.parametrize("prec", [B.PREC_FP64, B.PREC_FP32C])
+def test_batch_equals_per_sample_plans(hip_ctx, prec):
+    import torch
+    samples = _small_batch()
+    x, y, z = _concat(samples)
+    u = [torch.zeros_like(x) for _ in range(3)]
+    bt = B.Batch(hip_ctx, samples, 1, prec)
+    bt.run_torch(x, y, z, *u)
+    torch.cuda.synchronize()
+    got = np.stack([t.cpu().numpy() for t in u])
+    for s, off in zip(samples, bt.offsets):
+        ref = np.stack(B.line_sum(hip_ctx, s.x, s.y, s.z, s.loops, s.box, 1, prec))
+        # same per-atom device functions, same order: bit-identical
+        assert np.array_equal(got[:, off:off + s.n], ref)
+    assert bt.evals() == sum(s.n * s.segments() * 27 for s in samples)
+    bt.free()
+
+
+def test_batch_vs_oracle_and_graph_replays(hip_ctx):
+    import torch
+    samples = _small_batch(6, seed=3)
+    x, y, z = _concat(samples)
+    bt = B.Batch(hip_ctx, samples, 1, B.PREC_FP32C)
+    u = [torch.zeros_like(x) for _ in range(3)]
+    bt.run_torch(x, y, z, *u)
+    torch.cuda.synchronize()
+    first = np.stack([t.cpu().numpy() for t in u])
+    for rep in range(3):                     # replays with eager work in between
+        for t in u:
+            t.fill_(123.0)
+        junk = torch.randn(1000, device="cuda").sum().item()
+        bt.run_torch(x, y, z, *u)
+        torch.cuda.synchronize()
+        again = np.stack([t.cpu().numpy() for t in u])
+        assert np.array_equal(again, first), rep
+        assert np.isfinite(junk)
+    for s, off in zip(samples, bt.offsets):
+        idx = np.arange(0, s.n, max(1, s.n // 64))
+        ref = np.stack(oracle.line_sum(s.x[idx], s.y[idx], s.z[idx], s.loops, s.box, 1))
+        assert np.abs(first[:, off + idx] - ref).max() < 1e-5
+    bt.free()
+
+
+def test_autotune_and_tunings_agree(hip_ctx):
+    import torch
+    samples = _small_batch(8, seed=11)
+    x, y, z = _concat(samples)
+    bt = B.Batch(hip_ctx, samples, 1, B.PREC_FP32C)
+    u = [torch.zeros_like(x) for _ in range(3)]
+    bt.run_torch(x, y, z, *u)
+    torch.cuda.synchronize()
+    ref = np.stack([t.cpu().numpy() for t in u])
+    best, ms = bt.autotune(x, y, z, *u, stride=2)
+    assert best["block"] in (128, 256) and best["tile"] % best["block"] == 0 and ms > 0
+    for shape in ((64, 64, 37), (128, 512, 300), (256, 256, 2048), (best["block"], best["tile"], best["groups"])):
+        bt.set_tuning(*shape)
+        for t in u:
+            t.zero_()
+        bt.run_torch(x, y, z, *u)
+        torch.cuda.synchronize()
+        got = np.stack([t.cpu().numpy() for t in u])
+        assert np.array_equal(got, ref), shape
+    with pytest.raises(B.DiscoError):
+        bt.set_tuning(96, 96, 10)
+    bt.free()
+
+
+def test_batch_accumulate_and_empty_samples(hip_ctx):
+    import torch
+    samples = _small_batch(4, seed=5)
+    samples[1] = G.Sample("empty-loops", samples[1].x, samples[1].y, samples[1].z, [],
+                          samples[1].box, 1)
+    x, y, z = _concat(samples)
+    bt = B.Batch(hip_ctx, samples, 1, B.PREC_FP64)
+    u = [torch.full_like(x, 0.5) for _ in range(3)]
+    bt.run_torch(x, y, z, *u, accumulate=True)
+    torch.cuda.synchronize()
+    got = np.stack([t.cpu().numpy() for t in u])
+    off = bt.offsets[1]
+    assert np.all(got[:, off:off + samples[1].n] == 0.5)       # no loops: u unchanged
+    s = samples[2]
+    ref = np.stack(B.line_sum(hip_ctx, s.x, s.y, s.z, s.loops, s.box, 1, B.PREC_FP64))
+    o2 = bt.offsets[2]
+    assert np.abs(got[:, o2:o2 + s.n] - 0.5 - ref).max() < 1e-13
+    bt.free()

@@ -1,0 +1,124 @@
+"""The learned far shell: MFMA forward and backward kernels against the numpy
+FP64 reference (same bf16 rounding points), training, the feature and apply
+paths, and the validation / exact-fallback pipeline."""
+import numpy as np
+import pytest
+
+import mlp_ref as R
+from disco_amd import backend as B
+from disco_amd import geometry as G
+from disco_amd import surrogate as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sur(hip_ctx):
+    s = S.Surrogate(hip_ctx, seed=5)
+    yield s
+    s.free()
+
+
+def _rows(n, seed=0):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, 14)).astype(np.float32)
+    X[:, 10:14] = np.eye(4)[rng.integers(0, 4, n)]
+    T = (rng.normal(size=(n, 3)) * 0.3).astype(np.float32)
+    return X, T
+
+
+def test_forward_matches_reference(sur):
+    p = R.init_params(3)
+    sur.set_params(p)
+    assert np.array_equal(sur.params(), p)
+    X, _ = _rows(333)                          # not a multiple of 32
+    got = sur.forward(X)
+    want, _ = R.forward(p, X.astype(np.float64))
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print("forward rel err", err)
+    assert err < 5e-3
+
+
+def test_gradient_matches_reference(sur):
+    p = R.init_params(4)
+    sur.set_params(p)
+    X, T = _rows(257, seed=1)
+    loss, g = sur.loss_grad(X, T)
+    rl, rg = R.loss_grad(p, X.astype(np.float64), T.astype(np.float64))
+    assert abs(loss - rl) <= 1e-4 * rl
+    off = 0
+    for name, shp in R.SIZES:
+        n = int(np.prod(shp))
+        a, b = g[off:off + n], rg[off:off + n]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print(name, "grad rel err", rel)
+        assert rel < 2e-2, name
+        off += n
+
+
+def test_training_reduces_loss(hip_ctx):
+    s = S.Surrogate(hip_ctx, seed=9)
+    rng = np.random.default_rng(2)
+    X = rng.uniform(-1, 1, size=(20000, 14)).astype(np.float32)
+    T = np.stack([np.sin(2 * X[:, 0]) * X[:, 3], X[:, 1] * X[:, 2], np.cos(X[:, 9])], 1).astype(np.float32)
+    l0, _ = s.loss_grad(X[:4096], T[:4096])
+    s.train(X, T, epochs=15, batch=1024, lr=3e-3)
+    l1, _ = s.loss_grad(X[:4096], T[:4096])
+    print("loss", l0, "->", l1)
+    assert l1 < 0.05 * l0
+    s.free()
+
+
+def test_features_and_apply(hip_ctx, sur):
+    import torch
+    p = R.init_params(6)
+    sur.set_params(p)
+    c2 = G.config_c2(nloops=3)
+    infos = [S.shape_info(lp, g[1], "circle" if g[0] == 0 else "polygon")
+             for lp, g in zip(c2.loops, c2.meta["geo"])]
+    idx = np.arange(0, c2.n, 99991)
+    P = np.stack([c2.x[idx], c2.y[idx], c2.z[idx]], 1)
+    want = np.zeros((idx.size, 3))
+    for info in infos:
+        F = S.features(P, info, c2.box)
+        assert F.shape == (idx.size, 14)
+        assert np.allclose(F[:, 0], (P[:, 0] - info.c[0]) / c2.box[0], atol=1e-6)
+        want += S.scale(info, c2.box) * sur.forward(F)
+    x, y, z = (torch.from_numpy(np.ascontiguousarray(P[:, k])).cuda() for k in range(3))
+    u = [torch.zeros_like(x) for _ in range(3)]
+    sur.apply_torch(x, y, z, *u, infos, c2.box, chunk_rows=64)   # several chunks
+    got = np.stack([t.cpu().numpy() for t in u], 1)
+    assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max()
+
+
+def test_dipole_form_is_the_far_field_of_the_loop(hip_ctx):
+    """One loopon of the loop's area reproduces its far shell to O((R/r)^2)."""
+    import torch
+    c2 = G.config_c2(nloops=2)
+    lp = c2.loops[0]
+    rng = np.random.default_rng(1)
+    P = rng.uniform(-c2.box[0] / 2, c2.box[0] / 2, size=(64, 3))
+    x, y, z = (torch.from_numpy(np.ascontiguousarray(P[:, k])).cuda() for k in range(3))
+    ex = S.exact_far_shell_torch(hip_ctx, x, y, z, [lp], c2.box, 2, B.PREC_FP64)
+    dp = S.dipole_far_shell_torch(hip_ctx, x, y, z, [lp], c2.box, 2)
+    e = max(float((a - b).abs().max()) for a, b in zip(ex, dp))
+    m = max(float(a.abs().max()) for a in ex)
+    R_ = c2.meta["geo"][0][1]
+    print("dipole rel err", e / m, "R", R_)
+    assert e / m < 3 * (R_ / c2.box[0]) ** 2 + 1e-3
+
+
+def test_far_shell_falls_back_to_exact_when_untrained(hip_ctx, sur):
+    import torch
+    sur.set_params(R.init_params(8))                 # an untrained network
+    c2 = G.config_c2(nloops=2)
+    infos = [S.shape_info(lp, g[1], "circle" if g[0] == 0 else "polygon")
+             for lp, g in zip(c2.loops, c2.meta["geo"])]
+    idx = np.arange(0, c2.n, 7919)
+    x, y, z = (torch.from_numpy(a[idx].copy()).cuda() for a in (c2.x, c2.y, c2.z))
+    u = [torch.zeros_like(x) for _ in range(3)]
+    rep = S.far_shell(hip_ctx, sur, x, y, z, u, c2.loops, infos, c2.box, 2, tol=1e-9,
+                      n_valid=64)
+    assert not rep.used_surrogate and rep.max_err > 1e-9
+    ex = S.exact_far_shell_torch(hip_ctx, x, y, z, c2.loops, c2.box, 2)
+    assert max(float((a - b).abs().max()) for a, b in zip(u, ex)) == 0.0
