@@ -57,12 +57,20 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2 (default): the 4M-atom line-integral sample; c3: loopons; "
+                         "c4: 256 batched samples; c5: the far-shell surrogate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): atoms shard "
               "across GPUs with no exchange, and this build measures one GPU's share.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "c2":
+        from disco_amd import bench_workloads
+        bench_workloads.run(args)
+        return
 
     import torch
     from disco_amd import backend as B

@@ -1,0 +1,269 @@
+"""The other workloads of the benchmark (opt-in: ``bench.py --workload c3|c4|c5``).
+
+Each prints one JSON line in bench.py's format.  The default bench line (the
+4M-atom line-integral sample) stays in bench.py.
+
+* c3 -- loopons: BCC a0 = 2.8553, 60^3 cells (432,000 atoms), 20 circular loops
+        R = 40 A on {110} (2,001 loopons each), images +-1, FP32 pair terms.
+* c4 -- batched datagen: 256 BCC samples (20-60 cells, 1-16 loops of 16-128
+        segments: circles, rectangles, parallelograms), images +-1, one
+        persistent launch per pass replayed as a HIP graph, autotuned shape.
+* c5 -- learned far shell: FCC 100^3 (4M atoms), 200 loops as the default
+        sample, images +-4: exact line integral for +-1, surrogate (dipole far
+        shell + MLP) for the rest, validated against the exact far shell.
+"""
+from __future__ import annotations
+
+import json
+import time
+
+import numpy as np
+
+FLOP_LINE = 100.0
+FLOP_LOOPON = 58.0
+FP32_PEAK = 157.3
+BF16_PEAK = 2500.0
+
+
+def _emit(d):
+    print(json.dumps(d), flush=True)
+
+
+def _cpu_line(sample, stride, img=None):
+    import oracle
+    idx = np.arange(0, sample.n, stride)
+    img = sample.img if img is None else img
+    t0 = time.perf_counter()
+    oracle.line_sum(sample.x[idx], sample.y[idx], sample.z[idx], sample.loops, sample.box, img)
+    dt = time.perf_counter() - t0
+    ev = idx.size * sample.segments() * (2 * img + 1) ** 3
+    return ev / dt, oracle.threads(), f"every {stride}th atom ({idx.size}), images +-{img}, {dt:.1f} s"
+
+
+def run_c3(args):
+    import torch
+    import oracle
+    from . import backend as B
+    from . import geometry as G
+    s = G.config_c3()
+    dev = torch.device("cuda:0")
+    x, y, z = (torch.from_numpy(a).to(dev) for a in (s.x, s.y, s.z))
+    u = [torch.zeros_like(x) for _ in range(3)]
+    ctx = B.Context(0)
+    plan = B.Plan(ctx, s.loops, s.box, s.img, B.PREC_FP32C)
+    plan.profile(True)
+    evals = s.n * plan.evals_per_atom()
+    for _ in range(args.warmup):
+        plan.run_torch(x, y, z, *u)
+    torch.cuda.synchronize()
+    ms = []
+    for k in range(args.steps):
+        plan.run_torch(x, y, z, *u)
+        ms.append(plan.kernel_ms()[0])
+        print(f"[c3] step {k + 1}: {ms[-1]:.1f} ms", flush=True)
+    idx = torch.arange(0, s.n, 16, device=dev)
+    xs, ys, zs = x[idx].contiguous(), y[idx].contiguous(), z[idx].contiguous()
+    v = [torch.zeros_like(xs) for _ in range(3)]
+    p64 = B.Plan(ctx, s.loops, s.box, s.img, B.PREC_FP64)
+    p64.profile(True)
+    p64.run_torch(xs, ys, zs, *v)
+    ms64 = p64.kernel_ms()[0]
+    err = max(float((a[idx] - b).abs().max()) for a, b in zip(u, v))
+    cpu = None
+    if not args.no_cpu:
+        sub = np.arange(0, s.n, 128)
+        t0 = time.perf_counter()
+        oracle.loopon_sum(s.x[sub], s.y[sub], s.z[sub], s.loops, s.box, 1)
+        dt = time.perf_counter() - t0
+        cpu = {"value": sub.size * s.segments() * 27 / dt, "unit": "evals/s",
+               "cores": oracle.threads(), "kind": "port",
+               "sample": f"every 128th atom ({sub.size}), all 20 loops, +-1; {dt:.1f} s"}
+    k = float(np.mean(ms))
+    ach = evals * FLOP_LOOPON / (k * 1e-3) / 1e12
+    _emit({"metric": "atom·loopon·image evals/s (loopon method)", "value": evals / (k * 1e-3),
+           "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": k, "max_abs_err_A": err, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32 (f64 partial-sum flushes)", "data": "synthetic",
+           "config": {"workload": "BCC 60^3 (432,000 atoms), 20 loops x 2,001 loopons, images +-1",
+                      "evals_per_step": evals, "fp64_ms_per_16th": ms64},
+           "roofline": {"bound": "valu", "achieved": ach, "peak": FP32_PEAK, "unit": "TFLOP/s",
+                        "frac": ach / FP32_PEAK, "traffic": None, "kernel": "loopon_kernel<FP32C>",
+                        "kernel_ms": k, "flop_per_eval": FLOP_LOOPON},
+           "cpu_baseline": cpu})
+    ctx.close()
+
+
+def run_c4(args):
+    import torch
+    from . import backend as B
+    from . import geometry as G
+    t0 = time.time()
+    samples = G.config_c4()
+    print(f"[c4] {len(samples)} samples, {sum(s.n for s in samples)} atoms "
+          f"({time.time() - t0:.1f} s)", flush=True)
+    dev = torch.device("cuda:0")
+    cat = lambda k: torch.from_numpy(np.concatenate([getattr(s, k) for s in samples])).to(dev)
+    x, y, z = cat("x"), cat("y"), cat("z")
+    u = [torch.zeros_like(x) for _ in range(3)]
+    ctx = B.Context(0)
+    bt = B.Batch(ctx, samples, 1, B.PREC_FP32C)
+    evals = bt.evals()
+    default_shape = bt.tuning
+    bt.run_torch(x, y, z, *u)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    bt.run_torch(x, y, z, *u)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_default = e0.elapsed_time(e1)
+    t0 = time.perf_counter()
+    best, best_ms = bt.autotune(x, y, z, *u, stride=8)
+    tune_s = time.perf_counter() - t0
+    print(f"[c4] default {default_shape}: {ms_default:.1f} ms; tuned {best} "
+          f"(est. {best_ms:.1f} ms, {tune_s:.1f} s)", flush=True)
+    for _ in range(args.warmup):
+        bt.run_torch(x, y, z, *u)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(args.steps):
+        bt.run_torch(x, y, z, *u)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    # accuracy: FP64 sums of every 16th sample
+    errs = []
+    for k in range(0, len(samples), 16):
+        s = samples[k]
+        off = bt.offsets[k]
+        ref = B.line_sum(ctx, s.x, s.y, s.z, s.loops, s.box, 1, B.PREC_FP64)
+        got = [t[off:off + s.n].cpu().numpy() for t in u]
+        errs.append(max(float(np.abs(a - b).max()) for a, b in zip(got, ref)))
+    cpu = None
+    if not args.no_cpu:
+        v, c, desc = _cpu_line(samples[0], 64)
+        cpu = {"value": v, "unit": "evals/s", "cores": c, "kind": "port",
+               "sample": "sample 0 of the batch, " + desc}
+    ach = evals * FLOP_LINE / (ms * 1e-3) / 1e12
+    _emit({"metric": "atom·segment·image evals/s (batched samples)", "value": evals / (ms * 1e-3),
+           "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms, "max_abs_err_A": max(errs), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 image sums, f64 near cut)",
+           "data": "synthetic",
+           "config": {"workload": "256 BCC samples (20-60 cells, 1-16 loops, 16-128 segments), +-1",
+                      "atoms": int(sum(s.n for s in samples)), "evals_per_step": evals,
+                      "tuning": best, "default_shape_ms": ms_default, "autotune_s": tune_s},
+           "roofline": {"bound": "valu", "achieved": ach, "peak": FP32_PEAK, "unit": "TFLOP/s",
+                        "frac": ach / FP32_PEAK, "traffic": None,
+                        "kernel": "batch_line_kernel (far + near passes)", "kernel_ms": ms},
+           "cpu_baseline": cpu})
+    ctx.close()
+
+
+def run_c5(args):
+    import torch
+    from . import backend as B
+    from . import geometry as G
+    from . import surrogate as S
+    img = 4
+    t0 = time.time()
+    c = G.config_c2(seed=20245, nloops=200)
+    infos = [S.shape_info(lp, g[1], "circle" if g[0] == 0 else "polygon")
+             for lp, g in zip(c.loops, c.meta["geo"])]
+    print(f"[c5] workload {c.n} atoms, {len(c.loops)} loops, {c.segments()} segments "
+          f"({time.time() - t0:.1f} s)", flush=True)
+    dev = torch.device("cuda:0")
+    x, y, z = (torch.from_numpy(a).to(dev) for a in (c.x, c.y, c.z))
+    ctx = B.Context(0)
+    # ---- training (one-time), for both baselines
+    train = {}
+    surs = {}
+    for base in ("dipole", "rule7"):
+        t0 = time.perf_counter()
+        F, T, Sc, ex, bs = S.make_training_set(ctx, nloops=args.train_loops, atoms_per_loop=500,
+                                               img=img, seed=11, baseline=base)
+        t_data = time.perf_counter() - t0
+        n = F.shape[0]
+        ntr = int(0.9 * n)
+        sur = S.Surrogate(ctx, seed=1)
+        t0 = time.perf_counter()
+        loss = sur.train(F[:ntr], T[:ntr], epochs=args.epochs, batch=4096, lr=2e-3)
+        t_train = time.perf_counter() - t0
+        pred = sur.forward(F[ntr:])
+        err_net = float(np.abs((pred - T[ntr:]) * Sc[ntr:, None]).max())
+        err_base = float(np.abs(T[ntr:] * Sc[ntr:, None]).max())
+        rms_net = float(np.sqrt((((pred - T[ntr:]) * Sc[ntr:, None]) ** 2).mean()))
+        rms_base = float(np.sqrt(((T[ntr:] * Sc[ntr:, None]) ** 2).mean()))
+        train[base] = {"pairs": n, "data_s": t_data, "train_s": t_train, "epochs": args.epochs,
+                       "final_loss": loss, "held_out_max_err_A": err_net,
+                       "held_out_base_only_max_A": err_base, "held_out_rms_A": rms_net,
+                       "held_out_base_only_rms_A": rms_base}
+        surs[base] = sur
+        print(f"[c5] {base}: {n} pairs in {t_data:.1f} s, trained in {t_train:.1f} s; held-out "
+              f"per-pair max|err| {err_net:.3e} A (base only {err_base:.3e}), rms {rms_net:.3e} "
+              f"(base only {rms_base:.3e})", flush=True)
+    # ---- validation of the sample's far shell (rule7 + network), then timed steps
+    near = B.Plan(ctx, c.loops, c.box, 1, B.PREC_FP32C)
+    sur = surs["rule7"]
+    u = [torch.zeros_like(x) for _ in range(3)]
+    rep = S.far_shell(ctx, sur, x, y, z, u, c.loops, infos, c.box, img, tol=1e-5,
+                      exact_if_failed=False, baseline="rule7")
+    rep_d = S.far_shell(ctx, surs["dipole"], x, y, z, [torch.zeros_like(x) for _ in range(3)],
+                        c.loops, infos, c.box, img, tol=1e-5, exact_if_failed=False,
+                        baseline="dipole")
+    print(f"[c5] validation rule7+net: max|err| {rep.max_err:.3e} A (rule7 only "
+          f"{rep.base_only_err:.3e}) -> {'surrogate' if rep.used_surrogate else 'FALLBACK'}; "
+          f"dipole+net {rep_d.max_err:.3e} (dipole only {rep_d.base_only_err:.3e})", flush=True)
+    ts = []
+    for k in range(args.steps):
+        for t in u:
+            t.zero_()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        near.run_torch(x, y, z, *u, accumulate=True)
+        if rep.used_surrogate:
+            S.base_far_shell_torch(ctx, x, y, z, c.loops, c.box, img, "rule7", u=u,
+                                   accumulate=True)
+            sur.apply_torch(x, y, z, *u, infos, c.box)
+        else:
+            fp = B.Plan(ctx, c.loops, c.box, img, B.PREC_FP32C)
+            S.set_inner(fp, 1)
+            fp.run_torch(x, y, z, *u, accumulate=True)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t1)
+        print(f"[c5] step {k + 1}: {ts[-1] * 1e3:.1f} ms", flush=True)
+    t_sur = float(np.mean(ts))
+    # ---- the plain GPU sum over +-4 (timed on every 8th atom, scaled)
+    idx = torch.arange(0, c.n, 8, device=dev)
+    xs, ys, zs = x[idx].contiguous(), y[idx].contiguous(), z[idx].contiguous()
+    v = [torch.zeros_like(xs) for _ in range(3)]
+    plain = B.Plan(ctx, c.loops, c.box, img, B.PREC_FP32C)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    plain.run_torch(xs, ys, zs, *v)
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t1) * c.n / idx.numel()
+    err_total = max(float((a[idx] - b).abs().max()) for a, b in zip(u, v))
+    print(f"[c5] plain +-4 sum: {t_plain:.2f} s per sample (scaled from 1/8); this path "
+          f"{t_sur:.3f} s; max|du| vs plain (every 8th atom) {err_total:.3e} A", flush=True)
+    _emit({"metric": "ms per 4M-atom sample, images +-4 (far shell by surrogate)",
+           "value": t_sur * 1e3, "unit": "ms", "n_gpus": 1, "steps": args.steps,
+           "warmup": 0, "ms_per_step": t_sur * 1e3, "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 line +-1; f32 loopon rule far shell; bf16 MLP (fp32 accumulate)",
+           "data": "synthetic",
+           "config": {"workload": "FCC 100^3 (4M atoms), 200 loops, images +-4",
+                      "plain_gpu_ms": t_plain * 1e3, "speedup_vs_plain": t_plain / t_sur,
+                      "max_abs_err_vs_plain_A": err_total, "validation": rep.__dict__,
+                      "validation_dipole": rep_d.__dict__, "training": train,
+                      "plain_evals": c.n * c.segments() * (2 * img + 1) ** 3},
+           "roofline": None, "cpu_baseline": None})
+    ctx.close()
+
+
+def run(args):
+    if not hasattr(args, "train_loops"):
+        args.train_loops = 2000
+    if not hasattr(args, "epochs"):
+        args.epochs = 30
+    {"c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)

@@ -3,11 +3,17 @@
 The field of the loops over images (i, j, k) with max(|i|,|j|,|k|) in (1, img]
 is the far shell.  Its surrogate is
 
-    far(p) = sum_loops [ dipole_far(p) + scale * MLP(features(p, loop)) ]
+    far(p) = sum_loops [ base_far(p) + scale * MLP(features(p, loop)) ]
 
-* ``dipole_far`` -- the loop replaced by one loopon of its area at its centre
-  (the Kroupa term, Code/kroupa.hpp:52-73), summed exactly over the far shell
-  by the loopon kernels (``Plan`` with ``inner = 1``);
+* ``base_far`` -- the loop's cut replaced by a few weighted loopons (the Kroupa
+  term, Code/kroupa.hpp:52-73), summed exactly over the far shell by the
+  loopon kernels (``Plan`` with ``inner = 1``).  Two rules:
+  ``"dipole"``: one loopon of the loop's area at its centre (exact to first
+  order in R/r); ``"rule7"``: the centre plus a ring of six, with weights and
+  ring radius matching the cut's area and its second and fourth in-plane
+  moments -- exact through degree 5 for cuts whose moments are isotropic up
+  to rank 5 (circles, regular hexagons), so its far-shell error is
+  O((R/r)^6);
 * ``MLP`` -- 14 -> 128 -> 128 -> 128 -> 3 (SiLU), trained by the library's own
   GPU kernels (``disco_hip_surrogate_*``) on exact far shells generated by the
   line kernels, with ``scale = |b| (R / L)^4``.
@@ -115,6 +121,66 @@ def dipole_loopon(lp: B.LoopData) -> B.LoopData:
                       y=np.array([lp.cy]), z=np.array([lp.cz]))
 
 
+# Dunavant's 7-point degree-5 rule on a triangle (barycentric points, weights)
+_DW = np.array([0.225, 0.1323941527, 0.1323941527, 0.1323941527,
+                0.1259391805, 0.1259391805, 0.1259391805])
+_A1, _B1, _A2, _B2 = 0.0597158717, 0.4701420641, 0.7974269853, 0.1012865073
+_DB = np.array([[1 / 3, 1 / 3, 1 / 3], [_A1, _B1, _B1], [_B1, _A1, _B1], [_B1, _B1, _A1],
+                [_A2, _B2, _B2], [_B2, _A2, _B2], [_B2, _B2, _A2]])
+
+
+def area_moments(lp: B.LoopData):
+    """Area A, normal n and in-plane moments I2 = int r^2 dA, I4 = int r^4 dA of
+    the flat cut about its centre (fan triangles, exact degree-5 rule)."""
+    c = np.array([lp.cx, lp.cy, lp.cz])
+    V = np.stack([lp.vx, lp.vy, lp.vz], 1) - c
+    N = np.cross(np.roll(V, 1, axis=0), V).sum(0)
+    A = 0.5 * float(np.linalg.norm(N))
+    I2 = I4 = 0.0
+    for j in range(len(V)):
+        P0, P1 = V[j], V[(j + 1) % len(V)]
+        ar = 0.5 * float(np.linalg.norm(np.cross(P0, P1)))
+        pts = _DB[:, 1:2] * P0 + _DB[:, 2:3] * P1
+        r2 = (pts ** 2).sum(1)
+        I2 += ar * float((_DW * r2).sum())
+        I4 += ar * float((_DW * r2 * r2).sum())
+    return A, N / np.linalg.norm(N), I2, I4
+
+
+def rule7_loopons(lp: B.LoopData):
+    """The cut as 7 weighted loopons: w0 at the centre, w1 at six points on a
+    ring of radius rho, with w0 + 6 w1 = A, 6 w1 rho^2 = I2, 6 w1 rho^4 = I4
+    (for a disc: rho^2 = 2R^2/3, w0 = A/4, w1 = A/8 -- Radon's degree-5 rule).
+    Returns two loopon LoopData (centre, ring) sharing b, m, n."""
+    A, n, I2, I4 = area_moments(lp)
+    c = np.array([lp.cx, lp.cy, lp.cz])
+    rho = np.sqrt(I4 / I2)
+    w1 = I2 * I2 / (6.0 * I4)
+    w0 = A - 6.0 * w1
+    V0 = np.array([lp.vx[0], lp.vy[0], lp.vz[0]]) - c
+    e1 = V0 - np.dot(V0, n) * n
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(n, e1)
+    th = 2.0 * np.pi * np.arange(6) / 6.0
+    ring = c + rho * (np.cos(th)[:, None] * e1 + np.sin(th)[:, None] * e2)
+    b = np.array(lp.burgers())
+    bm = float(np.linalg.norm(b))
+    m = b / bm
+    centre = B.LoopData(phys=(bm, bm, bm, *m, *n, w0), line=False, x=c[:1].copy(),
+                        y=c[1:2].copy(), z=c[2:3].copy())
+    ringd = B.LoopData(phys=(bm, bm, bm, *m, *n, w1), line=False, x=ring[:, 0].copy(),
+                       y=ring[:, 1].copy(), z=ring[:, 2].copy())
+    return [centre, ringd]
+
+
+def base_loopons(loops, kind: str):
+    if kind == "dipole":
+        return [dipole_loopon(l) for l in loops]
+    if kind == "rule7":
+        return [q for l in loops for q in rule7_loopons(l)]
+    raise ValueError(f"unknown baseline {kind!r}")
+
+
 def features(points, info: LoopShapeInfo, box) -> np.ndarray:
     """Network inputs of atoms `points` (n x 3) for one loop (n x 14 float32)."""
     p = np.ascontiguousarray(points, np.float64)
@@ -219,16 +285,23 @@ def exact_far_shell_torch(ctx, x, y, z, loops, box, img, prec=B.PREC_FP32C):
     return u
 
 
-def dipole_far_shell_torch(ctx, x, y, z, loops, box, img, u=None, accumulate=False):
-    """Far shell of the loops' dipole terms, FP64 (exact for that term)."""
+def base_far_shell_torch(ctx, x, y, z, loops, box, img, kind="rule7", u=None,
+                         accumulate=False, prec=B.PREC_FP32C):
+    """Far shell of the loops' loopon rule (`kind`), summed by the loopon kernels."""
     import torch
     if u is None:
         u = [torch.zeros_like(x) for _ in range(3)]
-    plan = B.Plan(ctx, [dipole_loopon(l) for l in loops], box, img, B.PREC_FP64, kind="loopon")
+    plan = B.Plan(ctx, base_loopons(loops, kind), box, img, prec, kind="loopon")
     set_inner(plan, 1)
     plan.run_torch(x, y, z, *u, accumulate=accumulate)
     plan.free()
     return u
+
+
+def dipole_far_shell_torch(ctx, x, y, z, loops, box, img, u=None, accumulate=False):
+    """Far shell of the loops' dipole terms, FP64."""
+    return base_far_shell_torch(ctx, x, y, z, loops, box, img, "dipole", u, accumulate,
+                                B.PREC_FP64)
 
 
 # ------------------------------------------------------------------ training data
@@ -244,10 +317,12 @@ def random_fcc_loops(rng, nloops, a0=3.615, cells=100):
     return out
 
 
-def make_training_set(ctx, nloops=2000, atoms_per_loop=500, img=4, seed=11, a0=3.615, cells=100):
-    """Exact far shells of random (atom, loop) pairs, minus the dipole far shell,
-    divided by each loop's scale: (features n x 14, targets n x 3, scales n).
-    Computed on the GPU: each loop is one sample of a batched launch."""
+def make_training_set(ctx, nloops=2000, atoms_per_loop=500, img=4, seed=11, a0=3.615, cells=100,
+                      baseline="rule7"):
+    """Exact far shells of random (atom, loop) pairs, minus the baseline rule's
+    far shell, divided by each loop's scale: (features n x 14, targets n x 3,
+    scales n, exact n x 3, baseline n x 3).  Computed on the GPU: the exact
+    shells in one batched launch (each loop one sample), in FP64."""
     import torch
     rng = np.random.default_rng(seed)
     L = a0 * cells
@@ -271,12 +346,12 @@ def make_training_set(ctx, nloops=2000, atoms_per_loop=500, img=4, seed=11, a0=3
     torch.cuda.synchronize()
     exact = np.stack([t.cpu().numpy() for t in u], 1)
     bt.free()
-    # dipole far shell, loop by loop on its own atoms
+    # baseline far shell, loop by loop on its own atoms
     dip = np.zeros_like(exact)
     for k, (lp, _info) in enumerate(pairs):
         s = slice(k * atoms_per_loop, (k + 1) * atoms_per_loop)
-        ud = dipole_far_shell_torch(ctx, x[s].contiguous(), y[s].contiguous(),
-                                    z[s].contiguous(), [lp], box, img)
+        ud = base_far_shell_torch(ctx, x[s].contiguous(), y[s].contiguous(),
+                                  z[s].contiguous(), [lp], box, img, baseline, prec=B.PREC_FP64)
         dip[s] = np.stack([t.cpu().numpy() for t in ud], 1)
     S = np.concatenate(scales)
     targets = ((exact - dip) / S[:, None]).astype(np.float32)
@@ -288,17 +363,18 @@ def make_training_set(ctx, nloops=2000, atoms_per_loop=500, img=4, seed=11, a0=3
 class FarShellReport:
     used_surrogate: bool
     max_err: float          # surrogate vs exact far shell on the validation atoms (A)
-    dipole_only_err: float  # the same without the network (A)
+    base_only_err: float    # the same without the network (A)
+    baseline: str
     n_valid: int
     seconds_surrogate: float
     seconds_exact: float | None
 
 
 def far_shell(ctx, sur: Surrogate, x, y, z, u, loops, infos, box, img, tol=1e-5,
-              n_valid=4096, seed=3, exact_if_failed=True):
+              n_valid=4096, seed=3, exact_if_failed=True, baseline="rule7"):
     """u += far shell of the loops at the device atoms (x, y, z): the surrogate
-    (dipole + MLP) when its error on n_valid random atoms is within tol of the
-    exact far shell, else the exact far shell.  Returns a FarShellReport."""
+    (baseline rule + MLP) when its error on n_valid random atoms is within tol
+    of the exact far shell, else the exact far shell.  Returns a FarShellReport."""
     import torch
     n = x.numel()
     rng = np.random.default_rng(seed)
@@ -306,17 +382,18 @@ def far_shell(ctx, sur: Surrogate, x, y, z, u, loops, infos, box, img, tol=1e-5,
     xv, yv, zv = x[idx].contiguous(), y[idx].contiguous(), z[idx].contiguous()
     # judge: exact far shell on the validation atoms
     ex = exact_far_shell_torch(ctx, xv, yv, zv, loops, box, img)
-    dv = dipole_far_shell_torch(ctx, xv, yv, zv, loops, box, img)
+    dv = base_far_shell_torch(ctx, xv, yv, zv, loops, box, img, baseline)
     dip_only = max(float((e - d).abs().max()) for e, d in zip(ex, dv))
     sur.apply_torch(xv, yv, zv, *dv, infos, box)
     torch.cuda.synchronize()
     err = max(float((e - d).abs().max()) for e, d in zip(ex, dv))
     if err <= tol:
         t0 = time.perf_counter()
-        dipole_far_shell_torch(ctx, x, y, z, loops, box, img, u=u, accumulate=True)
+        base_far_shell_torch(ctx, x, y, z, loops, box, img, baseline, u=u, accumulate=True)
         sur.apply_torch(x, y, z, *u, infos, box)
         torch.cuda.synchronize()
-        return FarShellReport(True, err, dip_only, idx.numel(), time.perf_counter() - t0, None)
+        return FarShellReport(True, err, dip_only, baseline, idx.numel(),
+                              time.perf_counter() - t0, None)
     t_ex = None
     if exact_if_failed:
         t0 = time.perf_counter()
@@ -326,4 +403,4 @@ def far_shell(ctx, sur: Surrogate, x, y, z, u, loops, infos, box, img, tol=1e-5,
         torch.cuda.synchronize()
         plan.free()
         t_ex = time.perf_counter() - t0
-    return FarShellReport(False, err, dip_only, idx.numel(), 0.0, t_ex)
+    return FarShellReport(False, err, dip_only, baseline, idx.numel(), 0.0, t_ex)

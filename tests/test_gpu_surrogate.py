@@ -108,6 +108,39 @@ def test_dipole_form_is_the_far_field_of_the_loop(hip_ctx):
     assert e / m < 3 * (R_ / c2.box[0]) ** 2 + 1e-3
 
 
+def test_rule7_far_shell_is_within_the_bar(hip_ctx):
+    """The moment-matched 7-loopon rule reproduces each loop's far shell."""
+    import torch
+    c2 = G.config_c2(nloops=6)
+    rng = np.random.default_rng(3)
+    P = rng.uniform(-c2.box[0] / 2, c2.box[0] / 2, size=(256, 3))
+    x, y, z = (torch.from_numpy(np.ascontiguousarray(P[:, k])).cuda() for k in range(3))
+    ex = S.exact_far_shell_torch(hip_ctx, x, y, z, c2.loops, c2.box, 3, B.PREC_FP64)
+    r7 = S.base_far_shell_torch(hip_ctx, x, y, z, c2.loops, c2.box, 3, "rule7", prec=B.PREC_FP64)
+    dp = S.base_far_shell_torch(hip_ctx, x, y, z, c2.loops, c2.box, 3, "dipole", prec=B.PREC_FP64)
+    e7 = max(float((a - b).abs().max()) for a, b in zip(ex, r7))
+    ed = max(float((a - b).abs().max()) for a, b in zip(ex, dp))
+    print("far shell err: rule7", e7, "dipole", ed)
+    assert e7 < 1e-6 and e7 < ed / 100
+
+
+def test_trained_network_gains_over_its_baseline(hip_ctx):
+    """A network trained on exact far shells lowers the error of the dipole
+    baseline on held-out (atom, loop) pairs."""
+    F, T, Sc, ex, base = S.make_training_set(hip_ctx, nloops=240, atoms_per_loop=250, img=3,
+                                             seed=4, baseline="dipole")
+    n = F.shape[0]
+    ntr = int(0.85 * n)
+    s = S.Surrogate(hip_ctx, seed=2)
+    s.train(F[:ntr], T[:ntr], epochs=40, batch=2048, lr=2e-3)
+    pred = s.forward(F[ntr:])
+    err_net = np.sqrt((((pred - T[ntr:]) * Sc[ntr:, None]) ** 2).mean())
+    err_base = np.sqrt(((T[ntr:] * Sc[ntr:, None]) ** 2).mean())
+    print("held-out rms: baseline only", err_base, "with network", err_net)
+    assert err_net < 0.7 * err_base
+    s.free()
+
+
 def test_far_shell_falls_back_to_exact_when_untrained(hip_ctx, sur):
     import torch
     sur.set_params(R.init_params(8))                 # an untrained network
