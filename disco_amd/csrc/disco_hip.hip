@@ -52,6 +52,8 @@ struct DevMath {
     DISCO_HD static double fmax_d(double a, double b) { return fmax(a, b); }
     DISCO_HD static int exponent_d(double x) { return __builtin_amdgcn_frexp_exp(x); }
     DISCO_HD static double ldexp_d(double x, int e) { return __builtin_amdgcn_ldexp(x, e); }
+    // wave-uniform vote: true when the predicate holds on any active lane
+    DISCO_HD static bool any(bool p) { return __any(p); }
 };
 
 #undef DISCO_HD

@@ -129,63 +129,67 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
     float rA = M::rsqrt_f(la2);
     float RA = la2 * rA;
     float P = l1 * RA + (l1sq - (qx * last.bx + qy * last.by + qz * last.bz));
-    float rAl = rA * rl1;
 
     float X = 1.0f, Y = 0.0f;
     float swx = 0.f, swy = 0.f, swz = 0.f;
     float sax = 0.f, say = 0.f, saz = 0.f;
     float stx = 0.f, sty = 0.f, stz = 0.f;
+    // blocks of 16 segments: the complex product is renormalised between
+    // blocks (each scaled factor has modulus <= 4, so 16 of them stay finite)
+    for (int k0 = 0; k0 < nv; k0 += 16) {
+        const int k1 = nv < k0 + 16 ? nv : k0 + 16;
 #pragma unroll 2
-    for (int k = 0; k < nv; k++) {
-        const SegF& g = s[k];
-        const float px = g.bx - qx, py = g.by - qy, pz = g.bz - qz;       // B - p
-        const float lb2 = px * px + py * py + pz * pz;
-        const float rB = M::rsqrt_f(lb2);
-        const float RB = lb2 * rB;
+        for (int k = k0; k < k1; k++) {
+            const SegF& g = s[k];
+            const float px = g.bx - qx, py = g.by - qy, pz = g.bz - qz;   // B - p
+            const float lb2 = px * px + py * py + pz * pz;
+            const float rB = M::rsqrt_f(lb2);
+            const float RB = lb2 * rB;
+            const float rArB = rA * rB;
 
-        // solid-angle factor den + i num of triangle (c, A, B)
-        const float r1b = l1sq - (qx * g.bx + qy * g.by + qz * g.bz);
-        const float ab = ax * px + ay * py + az * pz;
-        float den = RB * P + RA * r1b + l1 * ab;
-        float num = -(qx * g.nx + qy * g.ny + qz * g.nz);
-        P = l1 * RB + r1b;
-        const float sc = rAl * rB;
-        rAl = rB * rl1;
-        den *= sc;
-        num *= sc;
-        const float Xn = X * den - Y * num;
-        Y = X * num + Y * den;
-        X = Xn;
-        if ((k & 15) == 15) {
-            const int e = M::exponent_f(M::fmax_f(M::fabs_f(X), M::fabs_f(Y)));
-            X = M::ldexp_f(X, -e);
-            Y = M::ldexp_f(Y, -e);
+            // solid-angle factor den + i num of triangle (c, A, B), scaled by
+            // 1 / (|c - p| |A - p| |B - p|)
+            const float r1b = l1sq - (qx * g.bx + qy * g.by + qz * g.bz);
+            const float ab = ax * px + ay * py + az * pz;
+            const float sc = rArB * rl1;
+            const float den = (RB * P + RA * r1b + l1 * ab) * sc;
+            const float num = -(qx * g.nx + qy * g.ny + qz * g.nz) * sc;
+            P = l1 * RB + r1b;
+            const float Xn = X * den - Y * num;
+            Y = X * num + Y * den;
+            X = Xn;
+
+            // segment terms; K0 = kn / kd, one reciprocal shared with J0's argument
+            const float sA = ax * g.tx + ay * g.ty + az * g.tz;
+            const float sB = sA + g.L;
+            const float RS = RA + RB;
+            float kn = g.L * (sA + sB) * rArB;
+            float kd = sB * RA + sA * RB;
+            const bool mixed = (sA < 0.0f) && (sB > 0.0f);      // atom beside the segment
+            if (M::any(mixed)) {
+                const float dx = ax - sA * g.tx, dy = ay - sA * g.ty, dz = az - sA * g.tz;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                const float knm = sB * rB - sA * rA;
+                kd = mixed ? d2 : kd;
+                kn = mixed ? knm : kn;
+            }
+            const float rr = M::rcp_f(RS * kd);
+            const float J0 = two_atanh_f<M>(g.L * kd * rr);
+            const float K0 = kn * RS * rr;
+            const float K1 = rA - rB;
+            const float dw = ax * g.wkx + ay * g.wky + az * g.wkz;   // cK (d.w) = cK (a.w)
+            const float al = dw * K0;
+            const float be = dw * (K1 - sA * K0);
+            swx += J0 * g.wjx; swy += J0 * g.wjy; swz += J0 * g.wjz;
+            sax += al * ax;    say += al * ay;    saz += al * az;
+            stx += be * g.tx;  sty += be * g.ty;  stz += be * g.tz;
+
+            ax = px; ay = py; az = pz;
+            RA = RB; rA = rB;
         }
-
-        // segment terms; K0 = kn / kd, one reciprocal shared with J0's argument
-        const float sA = ax * g.tx + ay * g.ty + az * g.tz;
-        const float sB = sA + g.L;
-        const float RS = RA + RB;
-        float kn = g.L * (sA + sB) * (rA * rB);
-        float kd = sB * RA + sA * RB;
-        if ((sA < 0.0f) && (sB > 0.0f)) {          // atom beside the segment
-            const float dx = ax - sA * g.tx, dy = ay - sA * g.ty, dz = az - sA * g.tz;
-            kd = dx * dx + dy * dy + dz * dz;
-            kn = sB * rB - sA * rA;
-        }
-        const float rr = M::rcp_f(RS * kd);
-        const float J0 = two_atanh_f<M>(g.L * kd * rr);
-        const float K0 = kn * RS * rr;
-        const float K1 = rA - rB;
-        const float dw = ax * g.wkx + ay * g.wky + az * g.wkz;   // cK (d.w) = cK (a.w)
-        const float al = dw * K0;
-        const float be = dw * (K1 - sA * K0);
-        swx += J0 * g.wjx; swy += J0 * g.wjy; swz += J0 * g.wjz;
-        sax += al * ax;    say += al * ay;    saz += al * az;
-        stx += be * g.tx;  sty += be * g.ty;  stz += be * g.tz;
-
-        ax = px; ay = py; az = pz;
-        RA = RB; rA = rB;
+        const int e = M::exponent_f(M::fmax_f(M::fabs_f(X), M::fabs_f(Y)));
+        X = M::ldexp_f(X, -e);
+        Y = M::ldexp_f(Y, -e);
     }
     const float fo = 2.0f * M::atan2_f(Y, X) * (float)kInv4Pi;
     ox += bx * fo + (swx - sax - stx);

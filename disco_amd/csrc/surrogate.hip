@@ -266,17 +266,18 @@ __global__ void pair_features_kernel(const double* __restrict__ x, const double*
     pair_features(x[a], y[a], z[a], loops[r % nl], L, X + (long long)r * kIn);
 }
 
-// u[a] += sum_l scale_l * Y[(a - a0) * nl + l, 0..2]
+// u[a] += gain * sum_l scale_l * Y[(a - a0) * nl + l, 0..2]
 __global__ void far_out_kernel(const float* __restrict__ Y, const disco_surrogate_loop* __restrict__ loops,
-                               int nl, double Lx, double Ly, double Lz, long long a0, int na,
-                               double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz)
+                               int nl, double Lx, double Ly, double Lz, double gain, long long a0,
+                               int na, double* __restrict__ ux, double* __restrict__ uy,
+                               double* __restrict__ uz)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= na) return;
     const double L[3] = {Lx, Ly, Lz};
     double sx = 0.0, sy = 0.0, sz = 0.0;
     for (int l = 0; l < nl; l++) {
-        const double s = pair_scale(loops[l], L);
+        const double s = gain * pair_scale(loops[l], L);
         const float* y = Y + ((long long)t * nl + l) * kOut;
         sx += s * y[0]; sy += s * y[1]; sz += s * y[2];
     }
@@ -299,6 +300,7 @@ struct disco_surrogate {
     bf16* Wb = nullptr;      // BF16 W2 | W3 | W4
     float* loss = nullptr;
     long long step = 0;
+    double gain = 1.0;       // output = gain * network
     // activation buffers for up to cap rows
     int cap = 0;
     float *X = nullptr, *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr, *Y = nullptr, *T = nullptr;
@@ -521,9 +523,18 @@ int disco_hip_surrogate_forward(disco_surrogate* s, const float* feats, int n, f
     DISCO_HIP_TRY(ctx, hipMemcpyAsync(Y.data(), s->Y, Y.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
     DISCO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     for (int r = 0; r < n; r++)
-        for (int k = 0; k < 3; k++) out[(size_t)r * 3 + k] = Y[(size_t)r * kOut + k];
+        for (int k = 0; k < 3; k++) out[(size_t)r * 3 + k] = (float)(s->gain * Y[(size_t)r * kOut + k]);
     return DISCO_OK;
 }
+
+int disco_hip_surrogate_set_gain(disco_surrogate* s, double gain)
+{
+    if (!s || !std::isfinite(gain)) return DISCO_ERR_ARG;
+    s->gain = gain;
+    return DISCO_OK;
+}
+
+double disco_hip_surrogate_get_gain(const disco_surrogate* s) { return s ? s->gain : 0.0; }
 
 int disco_hip_surrogate_loss_grad(disco_surrogate* s, const float* feats, const float* targets,
                                   int n, float* loss, float* grads)
@@ -635,7 +646,7 @@ int disco_hip_surrogate_apply(disco_surrogate* s, int natoms, const double* x, c
                            a0, dl, nloops, L[0], L[1], L[2], s->X, M);
         forward(s, M, false, st);
         hipLaunchKernelGGL(far_out_kernel, dim3(blocks(na, 256)), dim3(256), 0, st, s->Y, dl,
-                           nloops, L[0], L[1], L[2], a0, na, ux, uy, uz);
+                           nloops, L[0], L[1], L[2], s->gain, a0, na, ux, uy, uz);
     }
     const hipError_t e = hipGetLastError();
     (void)hipStreamSynchronize(st);

@@ -63,6 +63,9 @@ def _lib():
         L.disco_hip_surrogate_get_params.argtypes = [_vp, _fp]
         L.disco_hip_surrogate_set_params.argtypes = [_vp, _fp]
         L.disco_hip_surrogate_forward.argtypes = [_vp, _fp, ctypes.c_int, _fp]
+        L.disco_hip_surrogate_set_gain.argtypes = [_vp, ctypes.c_double]
+        L.disco_hip_surrogate_get_gain.argtypes = [_vp]
+        L.disco_hip_surrogate_get_gain.restype = ctypes.c_double
         L.disco_hip_surrogate_loss_grad.argtypes = [_vp, _fp, _fp, ctypes.c_int, _fp, _fp]
         L.disco_hip_surrogate_train.argtypes = [_vp, _fp, _fp, ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_int, ctypes.c_float, ctypes.c_uint, _fp]
@@ -237,9 +240,24 @@ class Surrogate:
                                                             g.ctypes.data_as(_fp)))
         return float(loss[0]), g
 
-    def train(self, feats, targets, epochs=20, batch=4096, lr=2e-3, seed=7) -> float:
+    @property
+    def gain(self) -> float:
+        return _lib().disco_hip_surrogate_get_gain(self._h)
+
+    def set_gain(self, g: float):
+        self.ctx.check(_lib().disco_hip_surrogate_set_gain(self._h, float(g)))
+
+    def train(self, feats, targets, epochs=20, batch=4096, lr=2e-3, seed=7,
+              normalize=True) -> float:
+        """Adam on the GPU.  With `normalize`, the network learns targets / RMS
+        and the surrogate's gain is set to that RMS."""
         f = np.ascontiguousarray(feats, np.float32)
-        t = np.ascontiguousarray(targets, np.float32)
+        t = np.ascontiguousarray(targets, np.float64)
+        g = float(np.sqrt((t ** 2).mean())) if normalize else 1.0
+        if not g > 0:
+            g = 1.0
+        t = np.ascontiguousarray(t / g, np.float32)
+        self.set_gain(g)
         loss = np.zeros(1, np.float32)
         self.ctx.check(_lib().disco_hip_surrogate_train(self._h, f.ctypes.data_as(_fp),
                                                         t.ctypes.data_as(_fp), f.shape[0],

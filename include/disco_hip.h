@@ -232,9 +232,15 @@ int disco_hip_surrogate_free(disco_surrogate *sur);
 /* FP32 master parameters (disco_hip_surrogate_num_params floats). */
 int disco_hip_surrogate_get_params(disco_surrogate *sur, float *out);
 int disco_hip_surrogate_set_params(disco_surrogate *sur, const float *in);
-/* Network output for n host feature rows (n x 14) into out (n x 3). */
+/* Output gain: the surrogate's output is gain * network (default 1); training
+ * on targets divided by their RMS and setting gain = that RMS keeps the
+ * network's outputs of order one. */
+int disco_hip_surrogate_set_gain(disco_surrogate *sur, double gain);
+double disco_hip_surrogate_get_gain(const disco_surrogate *sur);
+/* gain * network output for n host feature rows (n x 14) into out (n x 3). */
 int disco_hip_surrogate_forward(disco_surrogate *sur, const float *feats, int n, float *out);
-/* Mean squared error over n rows and its gradient (all parameters). */
+/* Mean squared error of the network (without the gain) over n rows and its
+ * gradient (all parameters). */
 int disco_hip_surrogate_loss_grad(disco_surrogate *sur, const float *feats,
                                   const float *targets, int n, float *loss, float *grads);
 /* Adam over a host data set (features n x 14, targets n x 3), cosine-decayed
@@ -242,7 +248,7 @@ int disco_hip_surrogate_loss_grad(disco_surrogate *sur, const float *feats,
 int disco_hip_surrogate_train(disco_surrogate *sur, const float *feats, const float *targets,
                               int n, int epochs, int batch, float lr, unsigned seed,
                               float *final_loss);
-/* u += sum over loops of scale * network(features) for device atoms; rows of
+/* u += sum over loops of gain * scale * network(features) for device atoms; rows of
  * (atom, loop) pairs are processed chunk_rows at a time (<= 0: 4M).
  * Synchronous on `stream`. */
 int disco_hip_surrogate_apply(disco_surrogate *sur, int natoms, const double *d_x,
