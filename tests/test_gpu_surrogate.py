@@ -61,10 +61,12 @@ def test_training_reduces_loss(hip_ctx):
     rng = np.random.default_rng(2)
     X = rng.uniform(-1, 1, size=(20000, 14)).astype(np.float32)
     T = np.stack([np.sin(2 * X[:, 0]) * X[:, 3], X[:, 1] * X[:, 2], np.cos(X[:, 9])], 1).astype(np.float32)
-    l0, _ = s.loss_grad(X[:4096], T[:4096])
-    s.train(X, T, epochs=15, batch=1024, lr=3e-3)
-    l1, _ = s.loss_grad(X[:4096], T[:4096])
-    print("loss", l0, "->", l1)
+    mse = lambda: float(((s.forward(X[:4096]) - T[:4096]) ** 2).mean())
+    l0 = mse()
+    s.train(X, T, epochs=15, batch=1024, lr=3e-3)          # targets normalised, gain set
+    l1 = mse()
+    print("mse", l0, "->", l1, "gain", s.gain)
+    assert abs(s.gain - np.sqrt((T.astype(np.float64) ** 2).mean())) < 1e-6 * s.gain
     assert l1 < 0.05 * l0
     s.free()
 
