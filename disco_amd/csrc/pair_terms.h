@@ -128,7 +128,7 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
     float la2 = ax * ax + ay * ay + az * az;
     float rA = M::rsqrt_f(la2);
     float RA = la2 * rA;
-    float P = l1 * RA + (l1sq - (qx * last.bx + qy * last.by + qz * last.bz));
+    float P = l1 * RA - (qx * ax + qy * ay + qz * az);        // l1 |A-p| + (c-p).(A-p)
 
     float X = 1.0f, Y = 0.0f;
     float swx = 0.f, swy = 0.f, swz = 0.f;
@@ -149,7 +149,7 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
 
             // solid-angle factor den + i num of triangle (c, A, B), scaled by
             // 1 / (|c - p| |A - p| |B - p|)
-            const float r1b = l1sq - (qx * g.bx + qy * g.by + qz * g.bz);
+            const float r1b = -(qx * px + qy * py + qz * pz);              // (c-p).(B-p)
             const float ab = ax * px + ay * py + az * pz;
             const float sc = rArB * rl1;
             const float den = (RB * P + RA * r1b + l1 * ab) * sc;
