@@ -136,7 +136,7 @@ def main():
     roofline = {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
                 "traffic": FAR_PASS_HBM_BYTES_PER_ATOM * s.n, "traffic_unit": "bytes",
-                "kernel": "line_far_f32_kernel", "kernel_ms": far,
+                "kernel": "line_kernel<PASS_FAR>", "kernel_ms": far,
                 "near_kernel_ms": float(np.mean(near_ms)),
                 "flop_per_eval": FLOP_PER_EVAL, "evals_per_launch": evals}
 

@@ -87,7 +87,7 @@ def run_c3(args):
            "config": {"workload": "BCC 60^3 (432,000 atoms), 20 loops x 2,001 loopons, images +-1",
                       "evals_per_step": evals, "fp64_ms_per_16th": ms64},
            "roofline": {"bound": "valu", "achieved": ach, "peak": FP32_PEAK, "unit": "TFLOP/s",
-                        "frac": ach / FP32_PEAK, "traffic": None, "kernel": "loopon_kernel<FP32C>",
+                        "frac": ach / FP32_PEAK, "traffic": None, "kernel": "loopon_kernel<DISCO_PREC_FP32C>",
                         "kernel_ms": k, "flop_per_eval": FLOP_LOOPON},
            "cpu_baseline": cpu})
     ctx.close()
