@@ -52,8 +52,9 @@ struct DevMath {
     DISCO_HD static double fmax_d(double a, double b) { return fmax(a, b); }
     DISCO_HD static int exponent_d(double x) { return __builtin_amdgcn_frexp_exp(x); }
     DISCO_HD static double ldexp_d(double x, int e) { return __builtin_amdgcn_ldexp(x, e); }
-    // wave-uniform vote: true when the predicate holds on any active lane
+    // wave-uniform votes over the active lanes
     DISCO_HD static bool any(bool p) { return __any(p); }
+    DISCO_HD static bool all(bool p) { return __all(p); }
 };
 
 #undef DISCO_HD
@@ -153,15 +154,26 @@ __device__ __forceinline__ void atom_loopon(double px, double py, double pz,
                                                   s3x, s3y, s3z);
                     } else {
                         const float fx = (float)qx, fy = (float)qy, fz = (float)qz;
+                        const float c2 = (float)L.c2;
+                        const float cb = (float)(L.c1 * L.b_n);
+                        const double r12 = L.c1 / L.c2;
+                        // wave-uniform: can any lane sit on a loopon of this image?
+                        const bool checked = DevMath::any(qx * qx + qy * qy + qz * qz <= L.rmax2);
                         for (int c0 = 0; c0 < L.np; c0 += kLooponChunk) {
                             const int cnt = min(kLooponChunk, L.np - c0);
                             float t1 = 0.f, t2 = 0.f, t3x = 0.f, t3y = 0.f, t3z = 0.f;
-                            loopon_image_f32<DevMath>(fx, fy, fz, pf + L.p0 + c0, cnt,
-                                                      (float)L.mx, (float)L.my, (float)L.mz,
-                                                      (float)L.nx, (float)L.ny, (float)L.nz,
-                                                      (float)L.b_n, (float)L.c1, (float)L.c2,
-                                                      t1, t2, t3x, t3y, t3z);
-                            s1 += t1; s2 += t2; s3x += t3x; s3y += t3y; s3z += t3z;
+                            if (checked)
+                                loopon_image_f32<DevMath, true>(
+                                    fx, fy, fz, pf + L.p0 + c0, cnt, c2 * (float)L.mx,
+                                    c2 * (float)L.my, c2 * (float)L.mz, (float)L.nx, (float)L.ny,
+                                    (float)L.nz, cb, t1, t2, t3x, t3y, t3z);
+                            else
+                                loopon_image_f32<DevMath, false>(
+                                    fx, fy, fz, pf + L.p0 + c0, cnt, c2 * (float)L.mx,
+                                    c2 * (float)L.my, c2 * (float)L.mz, (float)L.nx, (float)L.ny,
+                                    (float)L.nz, cb, t1, t2, t3x, t3y, t3z);
+                            s1 += r12 * t1; s2 += L.c1 * t2;
+                            s3x += t3x; s3y += t3y; s3z += t3z;
                         }
                     }
                 }

@@ -27,8 +27,9 @@
 //   * d.w = a.w and d K0 + t K1 = a K0 + t (K1 - s_A K0) since w = b x t is
 //     normal to t: the normal offset d is needed only for the mixed K0.
 // The Kroupa term is regrouped the same way: per image, with g = C1 / rho^3 and
-// h = C2 (m.rho)(n.rho) / rho^5, u_i = b_i [n_i sum g m.rho + m_i sum g n.rho
-// + sum rho_i (h - g m.n)], which is Code/kroupa.hpp:69-72 summed over loopons.
+// h = C2 (m.rho)(n.rho) / rho^5, u_i = b_i [n_i S1 + m_i S2 + S3_i],
+// S1 = sum g m.rho, S2 = sum g n.rho, S3 = sum rho (h - g m.n) -- which is
+// Code/kroupa.hpp:69-72 summed over loopons (C1, C2 folded into constants).
 //
 // The templates take the math operations as a policy class M (the HIP build
 // instantiates them with the gfx950 intrinsics in disco_hip.hip).
@@ -82,6 +83,8 @@ struct alignas(128) LooponLoop {
     double nx, ny, nz;
     double c1, c2;       // -k1 dS (1 - 2 nu), -3 k1 dS
     double b_n;          // m.n
+    double rmax2;        // (largest |loopon - centroid| + 1e-3)^2: a lane farther from
+                         // the centroid cannot sit on a loopon
     int p0, np;
 };
 
@@ -268,29 +271,40 @@ DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __rest
 }
 
 // ------------------------------------------------------------ loopons
-// Partial sums of one loop image over loopons [0, np): S1 = sum g m.rho,
-// S2 = sum g n.rho, S3 = sum rho (h - g m.n), rho = q - P_p.
-template <class M>
+// Partial sums of one loop image over loopons [0, np), rho = q - P_p:
+//   t1 = sum rho^-3 (m'.rho),  t2 = sum rho^-3 (n.rho),
+//   s3 = sum rho (m'.rho (n.rho) rho^-5 - cb rho^-3)
+// with m' = c2 m and cb = c1 (m.n), so that for the loop
+//   S1 = c1 t1 / c2, S2 = c1 t2, S3 = s3 (the sums of the header).
+// A pair with rho < 1e-10 adds nothing (Code/kroupa.hpp:57-67): CHECKED
+// evaluates that guard; the caller may pass CHECKED = false when no lane of
+// the wave lies within the loop's loopon radius (then no pair can be that close).
+template <class M, bool CHECKED>
 DISCO_HD void loopon_image_f32(float qx, float qy, float qz, const float4* __restrict__ P, int np,
                                float mx, float my, float mz, float nx, float ny, float nz,
-                               float b_n, float c1, float c2, float& s1, float& s2,
-                               float& s3x, float& s3y, float& s3z)
+                               float cb, float& t1, float& t2, float& s3x, float& s3y,
+                               float& s3z)
 {
+#pragma unroll 4
     for (int p = 0; p < np; p++) {
         const float4 v = P[p];
         const float rx = qx - v.x, ry = qy - v.y, rz = qz - v.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        const bool keep = r2 >= 1e-20f;
-        const float ri = M::rsqrt_f(keep ? r2 : 1.0f);
+        float r2 = rx * rx + ry * ry + rz * rz;
+        const bool keep = !CHECKED || r2 >= 1e-20f;
+        if (CHECKED) r2 = keep ? r2 : 1.0f;
+        const float ri = M::rsqrt_f(r2);
         const float ri2 = ri * ri;
-        const float ri3 = ri2 * ri;
+        float ri3 = ri2 * ri;
         const float mr = mx * rx + my * ry + mz * rz;
         const float nr = nx * rx + ny * ry + nz * rz;
-        const float g = keep ? c1 * ri3 : 0.0f;
-        const float h = keep ? c2 * (mr * nr) * (ri3 * ri2) : 0.0f;
-        s1 += g * mr;
-        s2 += g * nr;
-        const float co = h - g * b_n;
+        float h = (mr * nr) * (ri3 * ri2);
+        if (CHECKED) {
+            ri3 = keep ? ri3 : 0.0f;
+            h = keep ? h : 0.0f;
+        }
+        t1 += ri3 * mr;
+        t2 += ri3 * nr;
+        const float co = h - cb * ri3;
         s3x += co * rx; s3y += co * ry; s3z += co * rz;
     }
 }

@@ -160,15 +160,18 @@ inline std::string pack_loopon(const disco_loopon_loop* in, int nloops, LooponPa
         L.b_n = ph[3] * ph[6] + ph[4] * ph[7] + ph[5] * ph[8];
         L.p0 = (int)out.pd.size();
         L.np = lp.np;
+        double r2max = 0.0;
         for (int p = 0; p < lp.np; p++) {
             const double x = lp.x[p] - cx, y = lp.y[p] - cy, z = lp.z[p] - cz;
             if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z))
                 return "non-finite loopon";
+            r2max = std::fmax(r2max, x * x + y * y + z * z);
             double4 d; d.x = x; d.y = y; d.z = z; d.w = 0.0;
             float4 f; f.x = (float)x; f.y = (float)y; f.z = (float)z; f.w = 0.0f;
             out.pd.push_back(d);
             out.pf.push_back(f);
         }
+        L.rmax2 = (std::sqrt(r2max) + 1e-3) * (std::sqrt(r2max) + 1e-3);
         out.loopons += lp.np;
         out.loops.push_back(L);
     }
