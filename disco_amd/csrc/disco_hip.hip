@@ -39,7 +39,8 @@ using namespace disco;
 struct DevMath {
     DISCO_HD static float rsqrt_f(float x) { return __builtin_amdgcn_rsqf(x); }
     DISCO_HD static float rcp_f(float x) { return __builtin_amdgcn_rcpf(x); }
-    DISCO_HD static float log_f(float x) { return logf(x); }
+    // natural log from the hardware log2 (x >= 5/3 here: no denormal inputs)
+    DISCO_HD static float log_f(float x) { return __builtin_amdgcn_logf(x) * 0.693147181f; }
     DISCO_HD static float atan2_f(float y, float x) { return atan2f(y, x); }
     DISCO_HD static float fabs_f(float x) { return fabsf(x); }
     DISCO_HD static float fmax_f(float a, float b) { return fmaxf(a, b); }
@@ -67,23 +68,70 @@ constexpr int kBlock = 256;
 constexpr int kLooponChunk = 64;   // loopons per FP32 partial sum (flushed to FP64)
 
 // ------------------------------------------------------------------ per-atom sums
-// Near predicate of the FP32C mode: the loop image lies within the loop's near
-// slab (|h| < hn and in-plane radius^2 < rn2).  Both line passes evaluate it
-// with contraction off, so they take exactly the same decision for every
-// (atom, loop, image) and each triple is summed by exactly one of them.
-__device__ __forceinline__ bool near_slab(double qx, double qy, double qz, const LineLoop& L)
+// The lane in the frame (e1, e2, n) of loop L: Q = R (p - c), and an image's
+// offset Q - (di X + dj Y + dk Z) with the box edges X, Y, Z of the loop record
+// (wave-uniform, held in scalar registers).  Both line passes compute these
+// with contraction off, so the near predicate (|h| < hn and in-plane radius^2
+// < rn2) takes exactly the same decision in each and every (atom, loop, image)
+// is summed by exactly one of them.
+struct FrameAtom {
+    double Q1, Q2, Q3;
+};
+
+__device__ __forceinline__ void frame_atom(double pcx, double pcy, double pcz, const LineLoop& L,
+                                           FrameAtom& f)
 {
 #pragma clang fp contract(off)
-    const double h = qx * L.nx + qy * L.ny + qz * L.nz;
-    const double r2 = (qx * qx + qy * qy + qz * qz) - h * h;
-    return fabs(h) < L.hn && r2 < L.rn2;
+    f.Q1 = (pcx * L.e1x + pcy * L.e1y) + pcz * L.e1z;
+    f.Q2 = (pcx * L.e2x + pcy * L.e2y) + pcz * L.e2z;
+    f.Q3 = (pcx * L.nx + pcy * L.ny) + pcz * L.nz;
+}
+
+__device__ __forceinline__ bool frame_image(const FrameAtom& f, int di, int dj, int dk,
+                                            const LineLoop& L, double& q1, double& q2,
+                                            double& q3)
+{
+#pragma clang fp contract(off)
+    const double i = di, j = dj, k = dk;
+    q1 = f.Q1 - ((i * L.X1 + j * L.Y1) + k * L.Z1);
+    q2 = f.Q2 - ((i * L.X2 + j * L.Y2) + k * L.Z2);
+    q3 = f.Q3 - ((i * L.X3 + j * L.Y3) + k * L.Z3);
+    return fabs(q3) < L.hn && q1 * q1 + q2 * q2 < L.rn2;
 }
 
 // The three line passes over loops [l0, l1) and images [-img, img]^3 (kk
 // fastest) for the atom at p; loop, segment and image data are wave-uniform.
 enum { PASS_FP64 = 0, PASS_FAR = 1, PASS_NEAR = 2 };
 
-template <int PASS>
+// FP32 far pass of one flat loop: every lane evaluates every image (uniform
+// control flow keeps the segment table in scalar loads); near lanes discard
+// it.  Each image's field is turned to global axes in float and added to the
+// FP64 sums.
+template <int KIND>
+__device__ __forceinline__ void far_flat_loop(const FrameAtom& f, const LineLoop& L,
+                                              const SegP* __restrict__ s, int img, int inner,
+                                              double& sx, double& sy, double& sz)
+{
+    for (int di = -img; di <= img; di++)
+        for (int dj = -img; dj <= img; dj++)
+            for (int dk = -img; dk <= img; dk++) {
+                if (max(abs(di), max(abs(dj), abs(dk))) <= inner) continue;   // inner shell
+                double q1, q2, q3;
+                const bool near = frame_image(f, di, dj, dk, L, q1, q2, q3);
+                float fo, o1, o2, o3;
+                line_image_flat_f32<DevMath, KIND>((float)q1, (float)q2, (float)q3, s, L.nv, fo,
+                                                   o1, o2, o3);
+                if (!near) {
+                    sx += (double)(L.bf[0] * fo + (o1 * L.e1f[0] + o2 * L.e2f[0] + o3 * L.nf[0]));
+                    sy += (double)(L.bf[1] * fo + (o1 * L.e1f[1] + o2 * L.e2f[1] + o3 * L.nf[1]));
+                    sz += (double)(L.bf[2] * fo + (o1 * L.e1f[2] + o2 * L.e2f[2] + o3 * L.nf[2]));
+                }
+            }
+}
+
+// PASS_FAR sums loops of one kind (KIND), each kind in its own kernel so that
+// each is compiled (and its registers allocated) for its own form.
+template <int PASS, int KIND = LOOP_3D>
 __device__ __forceinline__ bool atom_line(double px, double py, double pz,
                                           const LineLoop* __restrict__ loops, int l0, int l1,
                                           const SegF* __restrict__ segf,
@@ -95,34 +143,48 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
     for (int l = l0; l < l1; l++) {
         const LineLoop& L = loops[l];
         const double pcx = px - L.cx, pcy = py - L.cy, pcz = pz - L.cz;
-        for (int di = -img; di <= img; di++)
-            for (int dj = -img; dj <= img; dj++)
-                for (int dk = -img; dk <= img; dk++) {
-                    if (max(abs(di), max(abs(dj), abs(dk))) <= inner) continue;   // inner shell
-                    const double qx = pcx - di * Lx, qy = pcy - dj * Ly, qz = pcz - dk * Lz;
-                    if (PASS == PASS_FP64) {
+        if (PASS == PASS_FP64) {
+            for (int di = -img; di <= img; di++)
+                for (int dj = -img; dj <= img; dj++)
+                    for (int dk = -img; dk <= img; dk++) {
+                        if (max(abs(di), max(abs(dj), abs(dk))) <= inner) continue;
+                        const double qx = pcx - di * Lx, qy = pcy - dj * Ly, qz = pcz - dk * Lz;
                         double ox = 0.0, oy = 0.0, oz = 0.0;
                         line_image_f64<DevMath>(qx, qy, qz, segd + L.seg0, L.nv, L.bx, L.by,
                                                 L.bz, ox, oy, oz);
                         sx += ox; sy += oy; sz += oz;
-                    } else if (PASS == PASS_FAR) {
-                        // every lane evaluates (uniform control flow keeps the
-                        // segment tables in scalar loads); near lanes discard it
+                    }
+            continue;
+        }
+        FrameAtom f;
+        frame_atom(pcx, pcy, pcz, L, f);
+        if (PASS == PASS_FAR && KIND != LOOP_3D) {
+            const SegP* sp = reinterpret_cast<const SegP*>(segf + L.seg0);
+            far_flat_loop<KIND>(f, L, sp, img, inner, sx, sy, sz);
+            continue;
+        }
+        for (int di = -img; di <= img; di++)
+            for (int dj = -img; dj <= img; dj++)
+                for (int dk = -img; dk <= img; dk++) {
+                    if (max(abs(di), max(abs(dj), abs(dk))) <= inner) continue;   // inner shell
+                    double q1, q2, q3;
+                    const bool near = frame_image(f, di, dj, dk, L, q1, q2, q3);
+                    const double qx = pcx - di * Lx, qy = pcy - dj * Ly, qz = pcz - dk * Lz;
+                    if (PASS == PASS_FAR) {
+                        // LOOP_3D: the global-axes FP32 form
                         float ox = 0.f, oy = 0.f, oz = 0.f;
                         line_image_f32<DevMath>((float)qx, (float)qy, (float)qz, segf + L.seg0,
                                                 L.nv, (float)L.bx, (float)L.by, (float)L.bz,
                                                 ox, oy, oz);
-                        if (!near_slab(qx, qy, qz, L)) {
+                        if (!near) {
                             sx += (double)ox; sy += (double)oy; sz += (double)oz;
                         }
-                    } else {
-                        if (near_slab(qx, qy, qz, L)) {
-                            double ox = 0.0, oy = 0.0, oz = 0.0;
-                            line_image_f64<DevMath>(qx, qy, qz, segd + L.seg0, L.nv, L.bx,
-                                                    L.by, L.bz, ox, oy, oz);
-                            sx += ox; sy += oy; sz += oz;
-                            any = true;
-                        }
+                    } else if (near) {
+                        double ox = 0.0, oy = 0.0, oz = 0.0;
+                        line_image_f64<DevMath>(qx, qy, qz, segd + L.seg0, L.nv, L.bx, L.by,
+                                                L.bz, ox, oy, oz);
+                        sx += ox; sy += oy; sz += oz;
+                        any = true;
                     }
                 }
     }
@@ -184,8 +246,9 @@ __device__ __forceinline__ void atom_loopon(double px, double py, double pz,
 }
 
 // ------------------------------------------------------------------ one-sample kernels
-// FP64 line sum (PASS_FP64) or the FP32C far pass (PASS_FAR): write u (or add).
-template <int PASS>
+// FP64 line sum (PASS_FP64) or the FP32C far pass over loops of kind KIND
+// (PASS_FAR): write u (or add).
+template <int PASS, int KIND>
 __global__ __launch_bounds__(kBlock) void line_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
@@ -196,8 +259,8 @@ __global__ __launch_bounds__(kBlock) void line_kernel(
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    atom_line<PASS>(x[i], y[i], z[i], loops, 0, nloops, segf, segd, Lx, Ly, Lz, img, inner, sx,
-                    sy, sz);
+    atom_line<PASS, KIND>(x[i], y[i], z[i], loops, 0, nloops, segf, segd, Lx, Ly, Lz, img, inner,
+                          sx, sy, sz);
     if (accumulate) { sx += ux[i]; sy += uy[i]; sz += uz[i]; }
     ux[i] = sx; uy[i] = sy; uz[i] = sz;
 }
@@ -246,12 +309,13 @@ __global__ __launch_bounds__(kBlock) void zero3_kernel(double* __restrict__ a,
 
 // ------------------------------------------------------------------ batched samples
 // One sample of a batch: its atoms [a0, a0 + n) of the concatenated arrays,
-// its loops [l0, l1) of the concatenated loop table, its box.
+// its loops [kofs[0], kofs[4]) of the concatenated loop table, those of kind k
+// being [kofs[k], kofs[k+1]), and its box.
 struct alignas(64) BatchSample {
     long long a0;
-    int n, l0, l1, pad;
+    int n;
+    int kofs[5];
     double Lx, Ly, Lz;
-    double pad2[2];
 };
 // A work item: atoms [start, start + count) of one sample.
 struct alignas(16) BatchItem {
@@ -268,7 +332,7 @@ __global__ void reset_counters_kernel(unsigned* __restrict__ c, int n)
 // Persistent batched line pass: each workgroup takes items from the queue
 // (one vector atomic per item) until every item is taken; items are sorted by
 // decreasing cost on the host (longest first), so the tail is short.
-template <int BLOCK, int PASS>
+template <int BLOCK, int PASS, int KIND>
 __global__ __launch_bounds__(BLOCK) void batch_line_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz,
@@ -286,11 +350,14 @@ __global__ __launch_bounds__(BLOCK) void batch_line_kernel(
         if (it >= nitems) return;
         const BatchItem I = items[it];
         const BatchSample& S = samples[I.sample];
+        const int l0 = PASS == PASS_FAR ? S.kofs[KIND] : S.kofs[0];
+        const int l1 = PASS == PASS_FAR ? S.kofs[KIND + 1] : S.kofs[4];
+        if (l0 == l1 && (accumulate || PASS == PASS_NEAR)) continue;   // nothing to add
         for (int a = threadIdx.x; a < I.count; a += BLOCK) {
             const long long i = S.a0 + I.start + a;
             double sx = 0.0, sy = 0.0, sz = 0.0;
-            const bool any = atom_line<PASS>(x[i], y[i], z[i], loops, S.l0, S.l1, segf, segd,
-                                             S.Lx, S.Ly, S.Lz, img, inner, sx, sy, sz);
+            const bool any = atom_line<PASS, KIND>(x[i], y[i], z[i], loops, l0, l1, segf, segd,
+                                                   S.Lx, S.Ly, S.Lz, img, inner, sx, sy, sz);
             if (PASS == PASS_NEAR) {
                 if (any) { ux[i] += sx; uy[i] += sy; uz[i] += sz; }
             } else {
@@ -312,6 +379,7 @@ struct disco_plan {
     int inner = -1;        // images with max(|i|,|j|,|k|) <= inner are skipped
     double L[3] = {0, 0, 0};
     double work = 0;       // segments (or loopons) per image
+    int kofs[5] = {0, 0, 0, 0, 0};   // line plans: loops of kind k are [kofs[k], kofs[k+1])
     void* d_loops = nullptr;
     void* d_f = nullptr;   // SegF / float4 table
     void* d_d = nullptr;   // SegD / double4 table
@@ -323,6 +391,7 @@ struct disco_plan {
 struct disco_batch {
     disco_ctx* ctx = nullptr;
     int prec = 0, img = 1, nsamples = 0;
+    bool kind_used[4] = {false, false, false, false};   // far-pass kernels to launch
     int inner = -1;
     std::vector<BatchSample> samples;     // host copy
     std::vector<double> seg_per_sample;   // segments of each sample
@@ -411,13 +480,24 @@ int launch(disco_plan* p, int n, const double* x, const double* y, const double*
         const SegF* sf = (const SegF*)p->d_f;
         const SegD* sd = (const SegD*)p->d_d;
         if (p->prec == DISCO_PREC_FP64) {
-            hipLaunchKernelGGL(line_kernel<PASS_FP64>, grid, block, 0, s, x, y, z, ux, uy, uz, n,
-                               lp, p->nloops, sf, sd, p->L[0], p->L[1], p->L[2], p->img,
-                               p->inner, accumulate);
+            hipLaunchKernelGGL((line_kernel<PASS_FP64, LOOP_3D>), grid, block, 0, s, x, y, z, ux,
+                               uy, uz, n, lp, p->nloops, sf, sd, p->L[0], p->L[1], p->L[2],
+                               p->img, p->inner, accumulate);
         } else {
-            hipLaunchKernelGGL(line_kernel<PASS_FAR>, grid, block, 0, s, x, y, z, ux, uy, uz, n,
-                               lp, p->nloops, sf, sd, p->L[0], p->L[1], p->L[2], p->img,
-                               p->inner, accumulate);
+            // far pass: one kernel per loop kind present; the first writes u
+            // (or adds, as asked), the others add
+            int acc = accumulate;
+            for (int k = 0; k < 4; k++) {
+                const int l0 = p->kofs[k], cnt = p->kofs[k + 1] - l0;
+                if (cnt == 0) continue;
+                auto kern = k == LOOP_3D ? line_kernel<PASS_FAR, LOOP_3D>
+                          : k == LOOP_FLAT ? line_kernel<PASS_FAR, LOOP_FLAT>
+                          : k == LOOP_GLIDE ? line_kernel<PASS_FAR, LOOP_GLIDE>
+                          : line_kernel<PASS_FAR, LOOP_PRISM>;
+                hipLaunchKernelGGL(kern, grid, block, 0, s, x, y, z, ux, uy, uz, n, lp + l0, cnt,
+                                   sf, sd, p->L[0], p->L[1], p->L[2], p->img, p->inner, acc);
+                acc = 1;
+            }
             if (prof) DISCO_HIP_TRY(ctx, hipEventRecord(p->ev[1], s));
             hipLaunchKernelGGL(line_near_kernel, grid, block, 0, s, x, y, z, ux, uy, uz, n, lp,
                                p->nloops, sd, p->L[0], p->L[1], p->L[2], p->img, p->inner);
@@ -449,7 +529,7 @@ int make_line_plan(disco_ctx* ctx, const disco_line_loop* loops, int nloops, dou
                    double Lz, int img, int prec, disco_plan** out, std::vector<double>* work)
 {
     LinePack pk;
-    const std::string e = pack_line(loops, nloops, pk);
+    const std::string e = pack_line(loops, nloops, Lx, Ly, Lz, pk);
     if (!e.empty()) return fail(ctx, DISCO_ERR_ARG, e);
     disco_plan* p = new (std::nothrow) disco_plan();
     if (!p) return fail(ctx, DISCO_ERR_OOM, "host allocation");
@@ -457,6 +537,7 @@ int make_line_plan(disco_ctx* ctx, const disco_line_loop* loops, int nloops, dou
     p->L[0] = Lx; p->L[1] = Ly; p->L[2] = Lz;
     p->nloops = (int)pk.loops.size();
     p->work = pk.segments;
+    std::memcpy(p->kofs, pk.kofs, sizeof(p->kofs));
     const int rc = upload_plan(ctx, p, pk.loops.data(), pk.loops.size() * sizeof(LineLoop),
                                pk.segf.data(), pk.segf.size() * sizeof(SegF), pk.segd.data(),
                                pk.segd.size() * sizeof(SegD));
@@ -586,7 +667,6 @@ std::vector<BatchItem> make_items(const disco_batch* b, int tile, int stride, do
     double ev = 0;
     for (int k = 0; k < b->nsamples; k += stride) {
         const BatchSample& S = b->samples[k];
-        if (S.l1 <= S.l0) continue;
         for (int st = 0; st < S.n; st += tile) {
             BatchItem it;
             it.sample = k; it.start = st; it.count = std::min(tile, S.n - st); it.pad = 0;
@@ -629,18 +709,31 @@ void batch_launch_block(const disco_batch* b, int groups, hipStream_t s, const d
     const SegF* sf = (const SegF*)b->d_segf;
     const SegD* sd = (const SegD*)b->d_segd;
     const dim3 grid((unsigned)groups), block(BLOCK);
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, b->d_counters, 2);
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, b->d_counters, 5);
     if (b->prec == DISCO_PREC_FP64) {
-        hipLaunchKernelGGL((batch_line_kernel<BLOCK, PASS_FP64>), grid, block, 0, s, x, y, z, ux,
-                           uy, uz, b->d_samples, b->d_items, b->nitems, b->d_counters, lp, sf,
-                           sd, b->img, b->inner, accumulate);
-    } else {
-        hipLaunchKernelGGL((batch_line_kernel<BLOCK, PASS_FAR>), grid, block, 0, s, x, y, z, ux,
-                           uy, uz, b->d_samples, b->d_items, b->nitems, b->d_counters, lp, sf,
-                           sd, b->img, b->inner, accumulate);
-        hipLaunchKernelGGL((batch_line_kernel<BLOCK, PASS_NEAR>), grid, block, 0, s, x, y, z,
-                           ux, uy, uz, b->d_samples, b->d_items, b->nitems, b->d_counters + 1,
+        hipLaunchKernelGGL((batch_line_kernel<BLOCK, PASS_FP64, LOOP_3D>), grid, block, 0, s, x,
+                           y, z, ux, uy, uz, b->d_samples, b->d_items, b->nitems, b->d_counters,
                            lp, sf, sd, b->img, b->inner, accumulate);
+    } else {
+        // far pass: one launch per loop kind present (queue counter k); the
+        // first writes u (or adds, as asked) for every sample, the others add
+        int acc = accumulate;
+        bool first = true;
+        for (int k = 0; k < 4; k++) {
+            if (!b->kind_used[k] && !(first && k == 3)) continue;
+            auto kern = k == LOOP_3D ? batch_line_kernel<BLOCK, PASS_FAR, LOOP_3D>
+                      : k == LOOP_FLAT ? batch_line_kernel<BLOCK, PASS_FAR, LOOP_FLAT>
+                      : k == LOOP_GLIDE ? batch_line_kernel<BLOCK, PASS_FAR, LOOP_GLIDE>
+                      : batch_line_kernel<BLOCK, PASS_FAR, LOOP_PRISM>;
+            hipLaunchKernelGGL(kern, grid, block, 0, s, x, y, z, ux, uy, uz, b->d_samples,
+                               b->d_items, b->nitems, b->d_counters + k, lp, sf, sd, b->img,
+                               b->inner, acc);
+            acc = 1;
+            first = false;
+        }
+        hipLaunchKernelGGL((batch_line_kernel<BLOCK, PASS_NEAR, LOOP_3D>), grid, block, 0, s, x,
+                           y, z, ux, uy, uz, b->d_samples, b->d_items, b->nitems,
+                           b->d_counters + 4, lp, sf, sd, b->img, b->inner, accumulate);
     }
 }
 
@@ -919,7 +1012,7 @@ int disco_hip_batch_plan(disco_ctx* ctx, const disco_sample* samples, int nsampl
         else if (S.nloops > DISCO_MAX_LOOPS) { free_batch(b); return fail(ctx, DISCO_ERR_TOO_LARGE, "too many loops"); }
         else if (!(std::isfinite(S.L[0]) && std::isfinite(S.L[1]) && std::isfinite(S.L[2]))) why = "non-finite box";
         LinePack pk;
-        if (why.empty()) why = pack_line(S.loops, S.nloops, pk);
+        if (why.empty()) why = pack_line(S.loops, S.nloops, S.L[0], S.L[1], S.L[2], pk);
         if (!why.empty()) {
             free_batch(b);
             return fail(ctx, DISCO_ERR_ARG, "sample " + std::to_string(k) + ": " + why);
@@ -927,12 +1020,13 @@ int disco_hip_batch_plan(disco_ctx* ctx, const disco_sample* samples, int nsampl
         BatchSample bs;
         std::memset(&bs, 0, sizeof(bs));
         bs.a0 = S.atom_offset; bs.n = S.n;
-        bs.l0 = (int)all.loops.size();
+        const int loop_base = (int)all.loops.size();
         const int seg_base = (int)all.segd.size();
         for (LineLoop L : pk.loops) { L.seg0 += seg_base; all.loops.push_back(L); }
         all.segd.insert(all.segd.end(), pk.segd.begin(), pk.segd.end());
         all.segf.insert(all.segf.end(), pk.segf.begin(), pk.segf.end());
-        bs.l1 = (int)all.loops.size();
+        for (int q = 0; q < 5; q++) bs.kofs[q] = loop_base + pk.kofs[q];
+        for (int q = 0; q < 4; q++) b->kind_used[q] = b->kind_used[q] || pk.kofs[q + 1] > pk.kofs[q];
         bs.Lx = S.L[0]; bs.Ly = S.L[1]; bs.Lz = S.L[2];
         b->samples.push_back(bs);
         b->seg_per_sample.push_back(pk.segments);

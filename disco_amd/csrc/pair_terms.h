@@ -64,16 +64,27 @@ struct alignas(16 * sizeof(R)) Seg {
 using SegF = Seg<float>;
 using SegD = Seg<double>;
 
-// One loop of a line plan.
-struct alignas(128) LineLoop {
-    double cx, cy, cz;   // apex
-    double bx, by, bz;   // Burgers vector
-    double nx, ny, nz;   // unit plane normal (near test)
-    double rn2;          // near test: in-plane radius^2 bound
-    double hn;           // near test: height bound
-    int seg0, nv;        // first segment in the tables, count
-    double pad[4];
+// One loop of a line plan (read with scalar loads: every field is wave-uniform).
+struct alignas(64) LineLoop {
+    double cx, cy, cz;      // apex
+    double bx, by, bz;      // Burgers vector
+    double nx, ny, nz;      // unit plane normal
+    double e1x, e1y, e1z;   // unit in-plane axes: (e1, e2, n) is the loop frame
+    double e2x, e2y, e2z;
+    double X1, X2, X3;      // box edges Lx ex, Ly ey, Lz ez in the loop frame
+    double Y1, Y2, Y3;
+    double Z1, Z2, Z3;
+    double rn2;             // near test: in-plane radius^2 bound
+    double hn;              // near test: height bound
+    float bf[3];            // b, e1, e2, n rounded to float (frame -> global axes
+    float e1f[3];           // of the FP32 image fields)
+    float e2f[3];
+    float nf[3];
+    int seg0, nv;           // first segment in the tables, count
+    int kind, pad0;         // LOOP_3D / LOOP_FLAT / LOOP_GLIDE / LOOP_PRISM
+    double pad[5];
 };
+static_assert(sizeof(LineLoop) == 320, "loop record size");
 
 // One loop of a loopon plan; loopons are stored relative to (cx, cy, cz).
 struct alignas(128) LooponLoop {
@@ -91,20 +102,39 @@ struct alignas(128) LooponLoop {
 template <typename R>
 struct Vec3 { R x, y, z; };
 
+// Segment of a flat loop in the loop's frame (e1, e2, n), e1 and e2 spanning
+// its plane: every vertex has zero height, so positions and tangents have two
+// components, and (A - c) x (B - c) is normal to the plane.  Same size as
+// Seg<float> (the FP32 table holds one layout or the other per loop).
+struct alignas(64) SegP {
+    float b1, b2;       // B - c in the plane
+    float t1, t2;       // unit tangent
+    float L;            // |B - A|
+    float n3;           // ((A - c) x (B - c)).n = A1 B2 - A2 B1
+    float w1, w2, w3;   // b x t in the frame
+    float pad[7];
+};
+static_assert(sizeof(SegP) == sizeof(Seg<float>), "one FP32 segment slot per segment");
+
+// FP32 layout of a loop's segments.  LOOP_3D: Seg<float> in global axes (a
+// loop whose vertices leave its plane by more than float resolution).  The
+// flat kinds use SegP: LOOP_FLAT any b, LOOP_GLIDE b in the plane (b x t is
+// normal: w1 = w2 = 0), LOOP_PRISM b along n (b x t in the plane: w3 = 0).
+enum { LOOP_3D = 0, LOOP_FLAT = 1, LOOP_GLIDE = 2, LOOP_PRISM = 3 };
+
 // ------------------------------------------------------------ atanh pieces
-// 2 atanh(z) for 0 <= z < 1.  Series z (1 + z^2/3 + ... + z^10/11) for
-// z < 1/4 (first omitted term z^12/13 < 4.6e-9 relative); log form above.
+// 2 atanh(z) for 0 <= z < 1.  For z < 1/4: z (2 + y p(y)), y = z^2, p the
+// degree-2 minimax fit of (2 atanh(z)/z - 2)/y on [0, 1/16] (relative error
+// 1.8e-8 in exact arithmetic, below float rounding); log form above.
 template <class M>
 DISCO_HD float two_atanh_f(float z)
 {
     if (z < 0.25f) {
         const float y = z * z;
-        float p = 1.0f / 11.0f;
-        p = p * y + 1.0f / 9.0f;
-        p = p * y + 1.0f / 7.0f;
-        p = p * y + 1.0f / 5.0f;
-        p = p * y + 1.0f / 3.0f;
-        return 2.0f * z + (2.0f * z * y) * p;
+        float p = 0.314437628f;
+        p = p * y + 0.39893204f;
+        p = p * y + 0.666677833f;
+        return z * (y * p + 2.0f);
     }
     return M::log_f((1.0f + z) * M::rcp_f(1.0f - z));
 }
@@ -141,7 +171,6 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
     // blocks (each scaled factor has modulus <= 4, so 16 of them stay finite)
     for (int k0 = 0; k0 < nv; k0 += 16) {
         const int k1 = nv < k0 + 16 ? nv : k0 + 16;
-#pragma unroll 2
         for (int k = k0; k < k1; k++) {
             const SegF& g = s[k];
             const float px = g.bx - qx, py = g.by - qy, pz = g.bz - qz;   // B - p
@@ -198,6 +227,119 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
     ox += bx * fo + (swx - sax - stx);
     oy += by * fo + (swy - say - sty);
     oz += bz * fo + (swz - saz - stz);
+}
+
+// ------------------------------------------------------------ line, FP32, flat loops
+// The same algebra in the loop's frame, where the lane sits at q = (q1, q2, h)
+// and every vertex at height 0: B - p = (B1 - q1, B2 - q2, -h), so dot
+// products with positions and tangents take two products plus a per-image
+// constant, and num = -(q.N) = -h n3.  The solid-angle terms carry 1/l1:
+// P' = P/l1, (c-p).(B-p)/l1 = h^2/l1 - (q/l1).(B-p), so den/l1 needs no l1.
+struct FlatWalk {
+    float a1, a2, RA, rA, P;                 // A - p in the plane, |A-p|, 1/|A-p|, P/l1
+    float X, Y;                              // running solid-angle product
+    float sw1, sw2, sw3;                     // sum J0 w
+    float sa1, sa2, sal;                     // sum al a (in-plane), sum al (a3 = -h)
+    float st1, st2;                          // sum be t
+};
+
+template <class M, int KIND>
+DISCO_HD void flat_step(const SegP& g, float q1, float q2, float h2, float u1, float u2,
+                        float uh2, float nh, float mh, FlatWalk& w)
+{
+    const float p1 = g.b1 - q1, p2 = g.b2 - q2;                 // B - p, height -h
+    const float lb2 = p1 * p1 + (p2 * p2 + h2);
+    const float rB = M::rsqrt_f(lb2);
+    const float RB = lb2 * rB;
+    const float rArB = w.rA * rB;
+
+    const float r1b = (uh2 - u1 * p1) - u2 * p2;                // (c-p).(B-p) / l1
+    const float ab = w.a1 * p1 + (w.a2 * p2 + h2);              // (A-p).(B-p)
+    const float den = (RB * w.P + (w.RA * r1b + ab)) * rArB;
+    const float num = (g.n3 * nh) * rArB;
+    w.P = RB + r1b;
+    const float Xn = w.X * den - w.Y * num;
+    w.Y = w.X * num + w.Y * den;
+    w.X = Xn;
+
+    const float sA = w.a1 * g.t1 + w.a2 * g.t2;
+    const float sB = sA + g.L;
+    const float s2 = sA + sB;
+    const float RS = w.RA + RB;
+    float kn = g.L * s2 * rArB;
+    float kd = sB * w.RA + sA * RB;
+    const bool mixed = M::fabs_f(s2) < g.L;                     // sA < 0 < sB
+    if (__builtin_expect(M::any(mixed), 0)) {                   // a branch, not selects
+        const float d1 = w.a1 - sA * g.t1, d2 = w.a2 - sA * g.t2;
+        const float dd = d1 * d1 + (d2 * d2 + h2);
+        const float knm = sB * rB - sA * w.rA;
+        kd = mixed ? dd : kd;
+        kn = mixed ? knm : kn;
+    }
+    const float rr = M::rcp_f(RS * kd);
+    const float J0 = two_atanh_f<M>(g.L * kd * rr);
+    const float K0 = kn * RS * rr;
+    const float K1 = w.rA - rB;
+    // d.w = a.w with a = (a1, a2, -h); LOOP_GLIDE: a.w = -h w3, the factor -h
+    // is applied once after the walk
+    float dw;
+    if (KIND == LOOP_GLIDE) dw = g.w3;
+    else if (KIND == LOOP_PRISM) dw = w.a1 * g.w1 + w.a2 * g.w2;
+    else dw = w.a1 * g.w1 + (w.a2 * g.w2 + mh * g.w3);
+    const float al = dw * K0;
+    const float be = dw * (K1 - sA * K0);
+    if (KIND != LOOP_GLIDE) { w.sw1 += J0 * g.w1; w.sw2 += J0 * g.w2; }
+    if (KIND != LOOP_PRISM) w.sw3 += J0 * g.w3;
+    w.sa1 += al * w.a1; w.sa2 += al * w.a2; w.sal += al;
+    w.st1 += be * g.t1; w.st2 += be * g.t2;
+
+    w.a1 = p1; w.a2 = p2;
+    w.RA = RB; w.rA = rB;
+}
+
+// Field of one image of a flat loop, in the loop frame: fo = Omega / 4 pi
+// (the caller adds b fo) and o = cJ sum J0 w - cK sum (al a + be t).
+template <class M, int KIND>
+DISCO_HD void line_image_flat_f32(float q1, float q2, float h, const SegP* __restrict__ s, int nv,
+                                  float& fo, float& o1, float& o2, float& o3)
+{
+    const float h2 = h * h;
+    const float l1sq = q1 * q1 + (q2 * q2 + h2);
+    const float rl1 = M::rsqrt_f(l1sq);
+    const float u1 = q1 * rl1, u2 = q2 * rl1, uh2 = h2 * rl1, nh = -h * rl1, mh = -h;
+
+    FlatWalk w;
+    const SegP& last = s[nv - 1];
+    w.a1 = last.b1 - q1;
+    w.a2 = last.b2 - q2;
+    const float la2 = w.a1 * w.a1 + (w.a2 * w.a2 + h2);
+    w.rA = M::rsqrt_f(la2);
+    w.RA = la2 * w.rA;
+    w.P = w.RA + (uh2 - (u1 * w.a1 + u2 * w.a2));              // (l1 |A-p| + (c-p).(A-p)) / l1
+    w.X = 1.0f; w.Y = 0.0f;
+    w.sw1 = w.sw2 = w.sw3 = 0.0f;
+    w.sa1 = w.sa2 = w.sal = 0.0f;
+    w.st1 = w.st2 = 0.0f;
+    // blocks of 16 segments, renormalised between blocks; two segments per
+    // iteration (fewer register copies of the carried A - p)
+    for (int k0 = 0; k0 < nv; k0 += 16) {
+        const int k1 = nv < k0 + 16 ? nv : k0 + 16;
+        int k = k0;
+        for (; k + 1 < k1; k += 2) {
+            flat_step<M, KIND>(s[k], q1, q2, h2, u1, u2, uh2, nh, mh, w);
+            flat_step<M, KIND>(s[k + 1], q1, q2, h2, u1, u2, uh2, nh, mh, w);
+        }
+        for (; k < k1; k++) flat_step<M, KIND>(s[k], q1, q2, h2, u1, u2, uh2, nh, mh, w);
+        const int e = M::exponent_f(M::fmax_f(M::fabs_f(w.X), M::fabs_f(w.Y)));
+        w.X = M::ldexp_f(w.X, -e);
+        w.Y = M::ldexp_f(w.Y, -e);
+    }
+    fo = 2.0f * M::atan2_f(w.Y, w.X) * (float)kInv4Pi;
+    float x1 = w.sa1 + w.st1, x2 = w.sa2 + w.st2, x3 = mh * w.sal;
+    if (KIND == LOOP_GLIDE) { x1 *= mh; x2 *= mh; x3 *= mh; }
+    o1 = (float)kCJ * w.sw1 - (float)kC3 * x1;
+    o2 = (float)kCJ * w.sw2 - (float)kC3 * x2;
+    o3 = (float)kCJ * w.sw3 - (float)kC3 * x3;
 }
 
 // ------------------------------------------------------------ line, FP64

@@ -11,6 +11,7 @@
 // the reference's --fp32 upload does (Code/backend_cuda.cu:676-685).
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -22,7 +23,8 @@
 namespace disco {
 
 struct LinePack {
-    std::vector<LineLoop> loops;
+    std::vector<LineLoop> loops;   // ordered by kind: loops [kofs[k], kofs[k+1]) have kind k
+    int kofs[5] = {0, 0, 0, 0, 0};
     std::vector<SegF> segf;
     std::vector<SegD> segd;
     double segments = 0;      // total segment count (work per atom-image)
@@ -43,7 +45,8 @@ struct LooponPack {
 inline double near_height(double bmag) { return std::fmax(3.0 * bmag, 6.0); }
 
 // Returns "" on success, else the reason the input is rejected.
-inline std::string pack_line(const disco_line_loop* in, int nloops, LinePack& out)
+inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, double Ly,
+                             double Lz, LinePack& out)
 {
     out = LinePack();
     out.work.assign(nloops, 0.0);
@@ -98,6 +101,62 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, LinePack& ou
         L.rn2 = (rmax + H) * (rmax + H);
         L.seg0 = (int)out.segd.size();
         L.nv = nv;
+
+        // loop frame: e1 along the in-plane offset of the vertex farthest from
+        // the apex, e2 = n x e1
+        int kf = 0;
+        double best = -1.0;
+        for (int k = 0; k < nv; k++) {
+            const double h = ax[k] * nx + ay[k] * ny + az[k] * nz;
+            const double r2 = ax[k] * ax[k] + ay[k] * ay[k] + az[k] * az[k] - h * h;
+            if (r2 > best) { best = r2; kf = k; }
+        }
+        {
+            const double h = ax[kf] * nx + ay[kf] * ny + az[kf] * nz;
+            double ex = ax[kf] - h * nx, ey = ay[kf] - h * ny, ez = az[kf] - h * nz;
+            const double en = std::sqrt(ex * ex + ey * ey + ez * ez);
+            L.e1x = ex / en; L.e1y = ey / en; L.e1z = ez / en;
+            L.e2x = ny * L.e1z - nz * L.e1y;
+            L.e2y = nz * L.e1x - nx * L.e1z;
+            L.e2z = nx * L.e1y - ny * L.e1x;
+        }
+        // FP32 layout: the frame form when the vertices sit in the plane to
+        // float resolution (their heights are then below the rounding of their
+        // in-plane coordinates) and every projected segment has a length
+        const double res = std::ldexp(std::fmax(rmax, 1.0), -24);
+        std::vector<double> p1(nv), p2(nv);
+        for (int k = 0; k < nv; k++) {
+            p1[k] = ax[k] * L.e1x + ay[k] * L.e1y + az[k] * L.e1z;
+            p2[k] = ax[k] * L.e2x + ay[k] * L.e2y + az[k] * L.e2z;
+        }
+        bool flat = hmax <= res;
+        for (int k = 0; k < nv && flat; k++) {
+            const int j = (k + nv - 1) % nv;
+            flat = std::hypot(p1[k] - p1[j], p2[k] - p2[j]) > res;
+        }
+        const double b1 = lp.b[0] * L.e1x + lp.b[1] * L.e1y + lp.b[2] * L.e1z;
+        const double b2 = lp.b[0] * L.e2x + lp.b[1] * L.e2y + lp.b[2] * L.e2z;
+        const double b3 = lp.b[0] * nx + lp.b[1] * ny + lp.b[2] * nz;
+        const double btiny = std::ldexp(bmag, -24);
+        L.kind = !flat ? LOOP_3D
+               : std::fabs(b3) <= btiny ? LOOP_GLIDE
+               : std::hypot(b1, b2) <= btiny ? LOOP_PRISM : LOOP_FLAT;
+        // box edges in the frame; float copies for the FP32 image fields
+        L.X1 = Lx * L.e1x; L.X2 = Lx * L.e2x; L.X3 = Lx * nx;
+        L.Y1 = Ly * L.e1y; L.Y2 = Ly * L.e2y; L.Y3 = Ly * ny;
+        L.Z1 = Lz * L.e1z; L.Z2 = Lz * L.e2z; L.Z3 = Lz * nz;
+        const double e1v[3] = {L.e1x, L.e1y, L.e1z}, e2v[3] = {L.e2x, L.e2y, L.e2z};
+        const double nv3[3] = {nx, ny, nz};
+        for (int i = 0; i < 3; i++) {
+            L.bf[i] = (float)lp.b[i];
+            L.e1f[i] = (float)e1v[i];
+            L.e2f[i] = (float)e2v[i];
+            L.nf[i] = (float)nv3[i];
+        }
+        const double fb3 = L.kind == LOOP_GLIDE ? 0.0 : b3;
+        const double fb1 = L.kind == LOOP_PRISM ? 0.0 : b1;
+        const double fb2 = L.kind == LOOP_PRISM ? 0.0 : b2;
+
         for (int k = 0; k < nv; k++) {
             const int j = (k + nv - 1) % nv;
             SegD d;
@@ -117,17 +176,41 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, LinePack& ou
             d.wjx = kCJ * wx; d.wjy = kCJ * wy; d.wjz = kCJ * wz;
             SegF f;
             std::memset(&f, 0, sizeof(f));
-            f.bx = (float)d.bx; f.by = (float)d.by; f.bz = (float)d.bz;
-            f.tx = (float)d.tx; f.ty = (float)d.ty; f.tz = (float)d.tz;
-            f.L = (float)d.L;
-            f.nx = (float)d.nx; f.ny = (float)d.ny; f.nz = (float)d.nz;
-            f.wkx = (float)d.wkx; f.wky = (float)d.wky; f.wkz = (float)d.wkz;
-            f.wjx = (float)d.wjx; f.wjy = (float)d.wjy; f.wjz = (float)d.wjz;
+            if (L.kind == LOOP_3D) {
+                f.bx = (float)d.bx; f.by = (float)d.by; f.bz = (float)d.bz;
+                f.tx = (float)d.tx; f.ty = (float)d.ty; f.tz = (float)d.tz;
+                f.L = (float)d.L;
+                f.nx = (float)d.nx; f.ny = (float)d.ny; f.nz = (float)d.nz;
+                f.wkx = (float)d.wkx; f.wky = (float)d.wky; f.wkz = (float)d.wkz;
+                f.wjx = (float)d.wjx; f.wjy = (float)d.wjy; f.wjz = (float)d.wjz;
+            } else {
+                SegP g;
+                std::memset(&g, 0, sizeof(g));
+                const double e1 = p1[k] - p1[j], e2 = p2[k] - p2[j];
+                const double len2 = std::hypot(e1, e2);
+                const double t1 = e1 / len2, t2 = e2 / len2;
+                g.b1 = (float)p1[k]; g.b2 = (float)p2[k];
+                g.t1 = (float)t1; g.t2 = (float)t2;
+                g.L = (float)len2;
+                g.n3 = (float)(p1[j] * p2[k] - p2[j] * p1[k]);
+                g.w1 = (float)(-fb3 * t2);
+                g.w2 = (float)(fb3 * t1);
+                g.w3 = (float)(fb1 * t2 - fb2 * t1);
+                std::memcpy(&f, &g, sizeof(f));
+            }
             out.segd.push_back(d);
             out.segf.push_back(f);
         }
         out.segments += nv;
         out.loops.push_back(L);
+    }
+    // one far-pass kernel per loop kind (each compiled for its own form)
+    std::stable_sort(out.loops.begin(), out.loops.end(),
+                     [](const LineLoop& a, const LineLoop& b) { return a.kind < b.kind; });
+    for (int k = 0; k <= 4; k++) {
+        int c = 0;
+        while (c < (int)out.loops.size() && out.loops[c].kind < k) c++;
+        out.kofs[k] = c;
     }
     return "";
 }

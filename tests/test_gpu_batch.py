@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 import oracle
+from loop_kinds import BOX, atoms_near, kind_loops
 from disco_amd import backend as B
 from disco_amd import geometry as G
 
@@ -110,4 +111,37 @@ def test_batch_accumulate_and_empty_samples(hip_ctx):
     ref = np.stack(B.line_sum(hip_ctx, s.x, s.y, s.z, s.loops, s.box, 1, B.PREC_FP64))
     o2 = bt.offsets[2]
     assert np.abs(got[:, o2:o2 + s.n] - 0.5 - ref).max() < 1e-13
+    bt.free()
+
+
+@pytest.mark.parametrize("prec", [B.PREC_FP64, B.PREC_FP32C])
+def test_batch_loop_kinds_and_loopless_sample_overwrite(hip_ctx, prec):
+    """Samples holding different loop forms (one far launch per form present)
+    and a sample without loops: with accumulate = 0 every atom's u is written
+    (zero for the loop-less sample), equal to one-sample plans."""
+    import torch
+    rng = np.random.default_rng(11)
+    samples = []
+    for forms in (("glide",), ("prism", "bent"), (), ("mixed", "glide")):
+        loops = sum((kind_loops(k, rng, 2) for k in forms), [])
+        x, y, z = atoms_near(loops, rng, nrand=700, per_loop=100) if loops else \
+            tuple(rng.uniform(0, 150, 500) for _ in range(3))
+        samples.append(G.Sample("kinds", x, y, z, loops, BOX, 1))
+    x, y, z = _concat(samples)
+    bt = B.Batch(hip_ctx, samples, 1, prec)
+    u = [torch.full_like(x, 7.0) for _ in range(3)]
+    for rep in range(2):                       # capture, then a graph replay
+        bt.run_torch(x, y, z, *u, accumulate=False)
+        torch.cuda.synchronize()
+    got = np.stack([t.cpu().numpy() for t in u])
+    for k, s in enumerate(samples):
+        o = bt.offsets[k]
+        part = got[:, o:o + s.n]
+        if not s.loops:
+            assert np.all(part == 0.0)
+            continue
+        ref = np.stack(B.line_sum(hip_ctx, s.x, s.y, s.z, s.loops, s.box, 1, prec))
+        assert np.abs(part - ref).max() < 1e-12
+        orc = np.stack(oracle.line_sum(s.x, s.y, s.z, s.loops, s.box, 1))
+        assert np.abs(part - orc).max() < (1e-10 if prec == B.PREC_FP64 else 1e-5)
     bt.free()

@@ -7,6 +7,7 @@ import pytest
 
 import oracle
 from conftest import golden
+from loop_kinds import BOX, atoms_near, kind_loops
 from disco_amd import backend as B
 from disco_amd import geometry as G
 
@@ -157,3 +158,25 @@ def test_invalid_arguments_raise(hip_ctx):
                       vy=np.array([0., 0, 10, 10]), vz=np.array([0., 0, 0, 5]))
     with pytest.raises(B.DiscoError):
         B.line_sum(hip_ctx, x, x, x, [bent], s.box, 1, B.PREC_FP64)
+
+
+@pytest.mark.parametrize("kind", ["glide", "prism", "mixed", "bent", "all"])
+def test_loop_kinds_vs_oracle(hip_ctx, kind):
+    """Each FP32 far-pass form (glide, prismatic and mixed b summed in the loop
+    frame; a loop off its plane summed in global axes) and a plan holding all of
+    them (one far kernel per form, accumulated), against the FP64 oracle, +-1
+    images, random atoms and atoms near the loops."""
+    rng = np.random.default_rng({"glide": 1, "prism": 2, "mixed": 3, "bent": 4, "all": 5}[kind])
+    if kind == "all":
+        loops = sum((kind_loops(k, rng, 2) for k in ("glide", "prism", "mixed", "bent")), [])
+        rng.shuffle(loops)
+    else:
+        loops = kind_loops(kind, rng)
+    x, y, z = atoms_near(loops, rng)
+    ref = np.stack(oracle.line_sum(x, y, z, loops, BOX, 1))
+    u64 = np.stack(B.line_sum(hip_ctx, x, y, z, loops, BOX, 1, B.PREC_FP64))
+    u32 = np.stack(B.line_sum(hip_ctx, x, y, z, loops, BOX, 1, B.PREC_FP32C))
+    e64, e32 = np.abs(u64 - ref).max(), np.abs(u32 - ref).max()
+    print(f"{kind}: fp64 {e64:.2e}  fp32c {e32:.2e}  |u|max {np.abs(ref).max():.2f}")
+    assert e64 < TOL64
+    assert e32 < TOL32
