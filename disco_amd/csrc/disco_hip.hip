@@ -99,6 +99,19 @@ __device__ __forceinline__ bool frame_image(const FrameAtom& f, int di, int dj, 
     return fabs(q3) < L.hn && q1 * q1 + q2 * q2 < L.rn2;
 }
 
+// Range of image indices along one axis that can hold a near image of the
+// loop for this lane: a near image has |q| < sqrt(rn2 + hn^2) < reach, so its
+// offset d L lies within reach of the apex offset pc along every axis.
+__device__ __forceinline__ void near_range(double pc, double iL, double reach, int img, int& lo,
+                                           int& hi)
+{
+    if (iL == 0.0) { lo = -img; hi = img; return; }
+    const double a = fmin(fmax(ceil((pc - reach) * iL), -img - 1.0), img + 1.0);
+    const double b = fmin(fmax(floor((pc + reach) * iL), -img - 1.0), img + 1.0);
+    lo = max((int)a, -img);
+    hi = min((int)b, img);
+}
+
 // The three line passes over loops [l0, l1) and images [-img, img]^3 (kk
 // fastest) for the atom at p; loop, segment and image data are wave-uniform.
 enum { PASS_FP64 = 0, PASS_FAR = 1, PASS_NEAR = 2 };
@@ -161,6 +174,27 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
         if (PASS == PASS_FAR && KIND != LOOP_3D) {
             const SegP* sp = reinterpret_cast<const SegP*>(segf + L.seg0);
             far_flat_loop<KIND>(f, L, sp, img, inner, sx, sy, sz);
+            continue;
+        }
+        if (PASS == PASS_NEAR) {
+            // only the images that can be near (usually none or one per loop)
+            int i0, i1, j0, j1, k0, k1;
+            near_range(pcx, L.iLx, L.reach, img, i0, i1);
+            near_range(pcy, L.iLy, L.reach, img, j0, j1);
+            near_range(pcz, L.iLz, L.reach, img, k0, k1);
+            for (int di = i0; di <= i1; di++)
+                for (int dj = j0; dj <= j1; dj++)
+                    for (int dk = k0; dk <= k1; dk++) {
+                        if (max(abs(di), max(abs(dj), abs(dk))) <= inner) continue;
+                        double q1, q2, q3;
+                        if (!frame_image(f, di, dj, dk, L, q1, q2, q3)) continue;
+                        const double qx = pcx - di * Lx, qy = pcy - dj * Ly, qz = pcz - dk * Lz;
+                        double ox = 0.0, oy = 0.0, oz = 0.0;
+                        line_image_f64<DevMath>(qx, qy, qz, segd + L.seg0, L.nv, L.bx, L.by,
+                                                L.bz, ox, oy, oz);
+                        sx += ox; sy += oy; sz += oz;
+                        any = true;
+                    }
             continue;
         }
         for (int di = -img; di <= img; di++)

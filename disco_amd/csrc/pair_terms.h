@@ -82,7 +82,9 @@ struct alignas(64) LineLoop {
     float nf[3];
     int seg0, nv;           // first segment in the tables, count
     int kind, pad0;         // LOOP_3D / LOOP_FLAT / LOOP_GLIDE / LOOP_PRISM
-    double pad[5];
+    double reach;           // > sqrt(rn2 + hn^2): a near image has its apex this close
+    double iLx, iLy, iLz;   // 1 / box lengths (0 for a zero length)
+    double pad;
 };
 static_assert(sizeof(LineLoop) == 320, "loop record size");
 
@@ -239,8 +241,7 @@ struct FlatWalk {
     float a1, a2, RA, rA, P;                 // A - p in the plane, |A-p|, 1/|A-p|, P/l1
     float X, Y;                              // running solid-angle product
     float sw1, sw2, sw3;                     // sum J0 w
-    float sa1, sa2, sal;                     // sum al a (in-plane), sum al (a3 = -h)
-    float st1, st2;                          // sum be t
+    float x1, x2, sal;                       // sum (al a + be t) in the plane, sum al (a3 = -h)
 };
 
 template <class M, int KIND>
@@ -290,8 +291,9 @@ DISCO_HD void flat_step(const SegP& g, float q1, float q2, float h2, float u1, f
     const float be = dw * (K1 - sA * K0);
     if (KIND != LOOP_GLIDE) { w.sw1 += J0 * g.w1; w.sw2 += J0 * g.w2; }
     if (KIND != LOOP_PRISM) w.sw3 += J0 * g.w3;
-    w.sa1 += al * w.a1; w.sa2 += al * w.a2; w.sal += al;
-    w.st1 += be * g.t1; w.st2 += be * g.t2;
+    w.x1 = (w.x1 + al * w.a1) + be * g.t1;                     // two FMAs each
+    w.x2 = (w.x2 + al * w.a2) + be * g.t2;
+    w.sal += al;
 
     w.a1 = p1; w.a2 = p2;
     w.RA = RB; w.rA = rB;
@@ -318,8 +320,7 @@ DISCO_HD void line_image_flat_f32(float q1, float q2, float h, const SegP* __res
     w.P = w.RA + (uh2 - (u1 * w.a1 + u2 * w.a2));              // (l1 |A-p| + (c-p).(A-p)) / l1
     w.X = 1.0f; w.Y = 0.0f;
     w.sw1 = w.sw2 = w.sw3 = 0.0f;
-    w.sa1 = w.sa2 = w.sal = 0.0f;
-    w.st1 = w.st2 = 0.0f;
+    w.x1 = w.x2 = w.sal = 0.0f;
     // blocks of 16 segments, renormalised between blocks; two segments per
     // iteration (fewer register copies of the carried A - p)
     for (int k0 = 0; k0 < nv; k0 += 16) {
@@ -335,7 +336,7 @@ DISCO_HD void line_image_flat_f32(float q1, float q2, float h, const SegP* __res
         w.Y = M::ldexp_f(w.Y, -e);
     }
     fo = 2.0f * M::atan2_f(w.Y, w.X) * (float)kInv4Pi;
-    float x1 = w.sa1 + w.st1, x2 = w.sa2 + w.st2, x3 = mh * w.sal;
+    float x1 = w.x1, x2 = w.x2, x3 = mh * w.sal;
     if (KIND == LOOP_GLIDE) { x1 *= mh; x2 *= mh; x3 *= mh; }
     o1 = (float)kCJ * w.sw1 - (float)kC3 * x1;
     o2 = (float)kCJ * w.sw2 - (float)kC3 * x2;

@@ -141,6 +141,10 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, d
         L.kind = !flat ? LOOP_3D
                : std::fabs(b3) <= btiny ? LOOP_GLIDE
                : std::hypot(b1, b2) <= btiny ? LOOP_PRISM : LOOP_FLAT;
+        L.reach = std::sqrt(L.rn2 + L.hn * L.hn) * (1.0 + 1e-12) + 1e-6;
+        L.iLx = Lx > 0 ? 1.0 / Lx : 0.0;
+        L.iLy = Ly > 0 ? 1.0 / Ly : 0.0;
+        L.iLz = Lz > 0 ? 1.0 / Lz : 0.0;
         // box edges in the frame; float copies for the FP32 image fields
         L.X1 = Lx * L.e1x; L.X2 = Lx * L.e2x; L.X3 = Lx * nx;
         L.Y1 = Ly * L.e1y; L.Y2 = Ly * L.e2y; L.Y3 = Ly * ny;
