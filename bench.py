@@ -34,8 +34,8 @@ FLOP_PER_EVAL = 100.0
 FP32_VECTOR_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 # HBM bytes per atom of the far pass, measured with rocprofv3 PMC counters
 # (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, each scaled by a plain copy of
-# known size): 193.8e6 B per dispatch over 4,000,000 atoms.
-FAR_PASS_HBM_BYTES_PER_ATOM = 48.45
+# known size, `--workload copy`): 193.9e6 B per dispatch over 4,000,000 atoms.
+FAR_PASS_HBM_BYTES_PER_ATOM = 48.49
 
 
 def cpu_baseline(sample, stride):
@@ -53,6 +53,29 @@ def cpu_baseline(sample, stride):
             "seconds": dt}
 
 
+def copy_reference(args):
+    """y.copy_(x) of 256 MiB, 3 dispatches: reads and writes 262,144 KiB each, the
+    known size that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    x = torch.randn(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(3):
+        y.copy_(x)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / 3
+    nbytes = 2 * x.numel() * 8
+    print(json.dumps({"metric": "device copy GB/s (counter calibration)", "value": nbytes / ms / 1e6,
+                      "unit": "GB/s", "n_gpus": 1, "steps": 3, "warmup": 0, "ms_per_step": ms,
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                      "dtype": "f64", "data": "synthetic",
+                      "config": {"workload": "y.copy_(x), 256 MiB read + 256 MiB written"}}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,9 +84,10 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "copy"],
                     help="c2 (default): the 4M-atom line-integral sample; c3: loopons; "
-                         "c4: 256 batched samples; c5: the far-shell surrogate")
+                         "c4: 256 batched samples; c5: the far-shell surrogate; copy: a plain "
+                         "256 MiB device copy (x3), the size reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): atoms shard "
@@ -71,6 +95,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "copy":
+        copy_reference(args)
+        return
     if args.workload != "c2":
         from disco_amd import bench_workloads
         bench_workloads.run(args)
@@ -136,7 +163,7 @@ def main():
     roofline = {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
                 "traffic": FAR_PASS_HBM_BYTES_PER_ATOM * s.n, "traffic_unit": "bytes",
-                "kernel": "line_kernel<PASS_FAR>", "kernel_ms": far,
+                "kernel": "line_kernel<PASS_FAR, LOOP_GLIDE>", "kernel_ms": far,
                 "near_kernel_ms": float(np.mean(near_ms)),
                 "flop_per_eval": FLOP_PER_EVAL, "evals_per_launch": evals}
 

@@ -239,6 +239,49 @@ __device__ __forceinline__ void atom_loopon(double px, double py, double pz,
         const LooponLoop& L = loops[l];
         const double pcx = px - L.cx, pcy = py - L.cy, pcz = pz - L.cz;
         double s1 = 0.0, s2 = 0.0, s3x = 0.0, s3y = 0.0, s3z = 0.0;
+        if (PREC == DISCO_PREC_FP32C && L.flat) {
+            // flat loop: the lane in the loop frame, loopons in the plane
+            const double Q1 = pcx * L.e1x + pcy * L.e1y + pcz * L.e1z;
+            const double Q2 = pcx * L.e2x + pcy * L.e2y + pcz * L.e2z;
+            const double Q3 = pcx * L.ux + pcy * L.uy + pcz * L.uz;
+            const float cb = (float)(L.c1 * L.b_n);
+            const double r12 = L.c1 / L.c2;
+            double f1 = 0.0, f2 = 0.0, f3 = 0.0;          // s3 in the frame
+            for (int di = -img; di <= img; di++)
+                for (int dj = -img; dj <= img; dj++)
+                    for (int dk = -img; dk <= img; dk++) {
+                        if (max(abs(di), max(abs(dj), abs(dk))) <= inner) continue;
+                        const double q1 = Q1 - (di * L.X1 + dj * L.Y1 + dk * L.Z1);
+                        const double q2 = Q2 - (di * L.X2 + dj * L.Y2 + dk * L.Z2);
+                        const double q3 = Q3 - (di * L.X3 + dj * L.Y3 + dk * L.Z3);
+                        const float g1 = (float)q1, g2 = (float)q2, h = (float)q3;
+                        const float h2 = h * h, m3h = L.m3 * h, hn = (float)L.nmag * h;
+                        const bool checked =
+                            DevMath::any(q1 * q1 + q2 * q2 + q3 * q3 <= L.rmax2);
+                        for (int c0 = 0; c0 < L.np; c0 += kLooponChunk) {
+                            const int cnt = min(kLooponChunk, L.np - c0);
+                            float t1 = 0.f, t2 = 0.f, a1 = 0.f, a2 = 0.f, ac = 0.f;
+                            if (checked)
+                                loopon_flat_f32<DevMath, true>(g1, g2, h2, pf + L.p0 + c0, cnt,
+                                                               L.m1, L.m2, m3h, hn, cb, t1, t2,
+                                                               a1, a2, ac);
+                            else
+                                loopon_flat_f32<DevMath, false>(g1, g2, h2, pf + L.p0 + c0, cnt,
+                                                                L.m1, L.m2, m3h, hn, cb, t1, t2,
+                                                                a1, a2, ac);
+                            s1 += r12 * t1;
+                            s2 += L.c1 * (double)(hn * t2);
+                            f1 += a1; f2 += a2; f3 += (double)(h * ac);
+                        }
+                    }
+            s3x = f1 * L.e1x + f2 * L.e2x + f3 * L.ux;
+            s3y = f1 * L.e1y + f2 * L.e2y + f3 * L.uy;
+            s3z = f1 * L.e1z + f2 * L.e2z + f3 * L.uz;
+            ax += L.bx * (L.nx * s1 + L.mx * s2 + s3x);
+            ay += L.by * (L.ny * s1 + L.my * s2 + s3y);
+            az += L.bz * (L.nz * s1 + L.mz * s2 + s3z);
+            continue;
+        }
         for (int di = -img; di <= img; di++)
             for (int dj = -img; dj <= img; dj++)
                 for (int dk = -img; dk <= img; dk++) {
@@ -586,7 +629,7 @@ int make_loopon_plan(disco_ctx* ctx, const disco_loopon_loop* loops, int nloops,
                      std::vector<double>* work)
 {
     LooponPack pk;
-    const std::string e = pack_loopon(loops, nloops, pk);
+    const std::string e = pack_loopon(loops, nloops, Lx, Ly, Lz, pk);
     if (!e.empty()) return fail(ctx, DISCO_ERR_ARG, e);
     disco_plan* p = new (std::nothrow) disco_plan();
     if (!p) return fail(ctx, DISCO_ERR_OOM, "host allocation");

@@ -98,8 +98,16 @@ struct alignas(128) LooponLoop {
     double b_n;          // m.n
     double rmax2;        // (largest |loopon - centroid| + 1e-3)^2: a lane farther from
                          // the centroid cannot sit on a loopon
+    // flat loops (loopons in the plane normal to n, to float resolution): the
+    // frame (e1, e2, n/|n|), the box edges in it, |n| and c2 m in it
+    double e1x, e1y, e1z, e2x, e2y, e2z, ux, uy, uz;
+    double X1, X2, X3, Y1, Y2, Y3, Z1, Z2, Z3;
+    double nmag;
+    float m1, m2, m3;
+    int flat;            // the FP32 loopon table holds frame coordinates (P1, P2, 0)
     int p0, np;
 };
+static_assert(sizeof(LooponLoop) == 384, "loopon loop record size");
 
 template <typename R>
 struct Vec3 { R x, y, z; };
@@ -449,6 +457,36 @@ DISCO_HD void loopon_image_f32(float qx, float qy, float qz, const float4* __res
         t2 += ri3 * nr;
         const float co = h - cb * ri3;
         s3x += co * rx; s3y += co * ry; s3z += co * rz;
+    }
+}
+
+// The same sums for a flat loop in its frame, the lane at (q1, q2, h): with
+// rho = (r1, r2, h), n.rho = |n| h for every loopon, so per loopon
+//   t1 += rho^-3 (m'.rho),  s2 += rho^-3,  co = rho^-3 (|n| h (m'.rho) rho^-2 - cb),
+//   (s31, s32, sco) += co (r1, r2, 1)
+// and the caller forms t2 = |n| h s2 and s3 = (s31, s32, h sco).  m3h = m'_3 h,
+// hn = |n| h, h2 = h^2.
+template <class M, bool CHECKED>
+DISCO_HD void loopon_flat_f32(float q1, float q2, float h2, const float4* __restrict__ P, int np,
+                              float m1, float m2, float m3h, float hn, float cb, float& t1,
+                              float& s2, float& s31, float& s32, float& sco)
+{
+#pragma unroll 4
+    for (int p = 0; p < np; p++) {
+        const float4 v = P[p];
+        const float r1 = q1 - v.x, r2 = q2 - v.y;
+        float rr = r1 * r1 + (r2 * r2 + h2);
+        const bool keep = !CHECKED || rr >= 1e-20f;
+        if (CHECKED) rr = keep ? rr : 1.0f;
+        const float ri = M::rsqrt_f(rr);
+        const float ri2 = ri * ri;
+        float ri3 = ri2 * ri;
+        if (CHECKED) ri3 = keep ? ri3 : 0.0f;
+        const float mr = m1 * r1 + (m2 * r2 + m3h);
+        t1 += ri3 * mr;
+        s2 += ri3;
+        const float co = ri3 * (hn * (mr * ri2) - cb);
+        s31 += co * r1; s32 += co * r2; sco += co;
     }
 }
 

@@ -219,7 +219,8 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, d
     return "";
 }
 
-inline std::string pack_loopon(const disco_loopon_loop* in, int nloops, LooponPack& out)
+inline std::string pack_loopon(const disco_loopon_loop* in, int nloops, double Lx, double Ly,
+                               double Lz, LooponPack& out)
 {
     out = LooponPack();
     out.work.assign(nloops, 0.0);
@@ -259,6 +260,45 @@ inline std::string pack_loopon(const disco_loopon_loop* in, int nloops, LooponPa
             out.pf.push_back(f);
         }
         L.rmax2 = (std::sqrt(r2max) + 1e-3) * (std::sqrt(r2max) + 1e-3);
+
+        // flat loop: every loopon within float resolution of the plane through
+        // the centroid normal to n; the FP32 table then holds (P.e1, P.e2, 0, 0)
+        const double nmag = std::sqrt(L.nx * L.nx + L.ny * L.ny + L.nz * L.nz);
+        bool flat = nmag > 0.0;
+        if (flat) {
+            L.nmag = nmag;
+            L.ux = L.nx / nmag; L.uy = L.ny / nmag; L.uz = L.nz / nmag;
+            const double a[3] = {std::fabs(L.ux) < 0.9 ? 1.0 : 0.0, std::fabs(L.ux) < 0.9 ? 0.0 : 1.0, 0.0};
+            double ex = L.uy * a[2] - L.uz * a[1], ey = L.uz * a[0] - L.ux * a[2],
+                   ez = L.ux * a[1] - L.uy * a[0];
+            const double en = std::sqrt(ex * ex + ey * ey + ez * ez);
+            L.e1x = ex / en; L.e1y = ey / en; L.e1z = ez / en;
+            L.e2x = L.uy * L.e1z - L.uz * L.e1y;
+            L.e2y = L.uz * L.e1x - L.ux * L.e1z;
+            L.e2z = L.ux * L.e1y - L.uy * L.e1x;
+            const double res = std::ldexp(std::fmax(std::sqrt(r2max), 1.0), -24);
+            const size_t base = out.pd.size() - (size_t)lp.np;
+            for (int p = 0; p < lp.np && flat; p++) {
+                const double4& d = out.pd[base + p];
+                flat = std::fabs(d.x * L.ux + d.y * L.uy + d.z * L.uz) <= res;
+            }
+            if (flat) {
+                for (int p = 0; p < lp.np; p++) {
+                    const double4& d = out.pd[base + p];
+                    float4& f = out.pf[base + p];
+                    f.x = (float)(d.x * L.e1x + d.y * L.e1y + d.z * L.e1z);
+                    f.y = (float)(d.x * L.e2x + d.y * L.e2y + d.z * L.e2z);
+                    f.z = 0.0f; f.w = 0.0f;
+                }
+                L.X1 = Lx * L.e1x; L.X2 = Lx * L.e2x; L.X3 = Lx * L.ux;
+                L.Y1 = Ly * L.e1y; L.Y2 = Ly * L.e2y; L.Y3 = Ly * L.uy;
+                L.Z1 = Lz * L.e1z; L.Z2 = Lz * L.e2z; L.Z3 = Lz * L.uz;
+                L.m1 = (float)(L.c2 * (L.mx * L.e1x + L.my * L.e1y + L.mz * L.e1z));
+                L.m2 = (float)(L.c2 * (L.mx * L.e2x + L.my * L.e2y + L.mz * L.e2z));
+                L.m3 = (float)(L.c2 * (L.mx * L.ux + L.my * L.uy + L.mz * L.uz));
+            }
+        }
+        L.flat = flat ? 1 : 0;
         out.loopons += lp.np;
         out.loops.push_back(L);
     }

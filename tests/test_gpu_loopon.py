@@ -61,3 +61,58 @@ def test_atom_on_loopon_contributes_nothing(hip_ctx):
         u = np.stack(B.loopon_sum(hip_ctx, x, y, z, [lp], (1e3, 1e3, 1e3), 0, prec))
         assert np.all(np.isfinite(u))
         assert np.abs(u - ref).max() < tol
+
+
+def _disk_loopons(rng, n, c, R, m, nscale=1.0, bent=0.0):
+    e1 = np.cross(n, [1.0, 0, 0] if abs(n[0]) < 0.9 else [0, 1.0, 0])
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(n, e1)
+    g = np.arange(-R, R + 1e-9, 1.6)
+    u, v = np.meshgrid(g, g)
+    keep = u ** 2 + v ** 2 <= R * R
+    u, v = u[keep], v[keep]
+    h = bent * rng.uniform(-1, 1, u.size)          # off the plane (3D form) when bent > 0
+    P = c + u[:, None] * e1 + v[:, None] * e2 + h[:, None] * n
+    nn = n * nscale
+    return B.LoopData(phys=(2.5, 2.5, 2.5, *m, *nn, 1.6 * 1.6), line=False,
+                      x=P[:, 0].copy(), y=P[:, 1].copy(), z=P[:, 2].copy())
+
+
+@pytest.mark.parametrize("form", ["glide", "prism", "scaled_n", "bent"])
+def test_loopon_forms_vs_oracle(hip_ctx, form):
+    """Loopon loops in the loop frame (glide m in the plane, prismatic m along
+    n, a non-unit n) and loopons off their plane by up to 0.3 A (the global-axes
+    form), FP32 against the FP64 oracle on random atoms and atoms near the loops."""
+    rng = np.random.default_rng({"glide": 1, "prism": 2, "scaled_n": 3, "bent": 4}[form])
+    box = (120.0, 130.0, 140.0)
+    loops, near = [], []
+    for k in range(3):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        e = np.cross(n, rng.normal(size=3))
+        e /= np.linalg.norm(e)
+        m = n if form == "prism" else e
+        c = rng.uniform(30, 90, 3)
+        lp = _disk_loopons(rng, n, c, rng.uniform(8, 20), m,
+                           nscale=1.7 if form == "scaled_n" else 1.0,
+                           bent=0.3 if form == "bent" else 0.0)
+        loops.append(lp)
+        # atoms near the loop but >= 1.2 A off its plane (a lattice plane next to
+        # a loop on a mid-plane): the loopon sum is a far-field discretisation
+        e2 = np.cross(n, e)
+        r = rng.uniform(0, 25, 150)
+        ph = rng.uniform(0, 2 * np.pi, 150)
+        h = rng.choice([-1, 1], 150) * rng.uniform(1.2, 8, 150)
+        near.append(c + r[:, None] * (np.cos(ph)[:, None] * e + np.sin(ph)[:, None] * e2)
+                    + h[:, None] * n)
+    p = np.concatenate([rng.uniform(0, 1, (500, 3)) * np.array(box)] + near)
+    P = np.concatenate([np.stack([lp.x, lp.y, lp.z], 1) for lp in loops])
+    dmin = np.sqrt(((p[:, None, :] - P[None, :, :]) ** 2).sum(-1)).min(1)
+    p = p[dmin >= 1.2]                     # no atom on top of a loopon
+    x, y, z = p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy()
+    ref = np.stack(oracle.loopon_sum(x, y, z, loops, box, 1))
+    for prec, tol in ((B.PREC_FP64, 1e-10), (B.PREC_FP32C, 1e-5)):
+        u = np.stack(B.loopon_sum(hip_ctx, x, y, z, loops, box, 1, prec))
+        err = np.abs(u - ref).max()
+        print(f"{form} prec {prec}: {err:.3e}  |u|max {np.abs(ref).max():.2f}")
+        assert err < tol
