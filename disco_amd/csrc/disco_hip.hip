@@ -47,6 +47,23 @@ struct DevMath {
     DISCO_HD static int exponent_f(float x) { return __builtin_amdgcn_frexp_expf(x); }
     DISCO_HD static float ldexp_f(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
     DISCO_HD static double sqrt_d(double x) { return sqrt(x); }
+    // 1/sqrt(x) and 1/x: the hardware estimates refined by two Newton steps
+    // (quadratic convergence from ~2^-22 to rounding level)
+    DISCO_HD static double rsqrt_d(double x)
+    {
+        const double h = 0.5 * x;
+        double y = __builtin_amdgcn_rsq(x);
+        y = fma(y, fma(-h * y, y, 0.5), y);
+        y = fma(y, fma(-h * y, y, 0.5), y);
+        return y;
+    }
+    DISCO_HD static double rcp_d(double x)
+    {
+        double y = __builtin_amdgcn_rcp(x);
+        y = fma(y, fma(-x, y, 1.0), y);
+        y = fma(y, fma(-x, y, 1.0), y);
+        return y;
+    }
     DISCO_HD static double log1p_d(double x) { return log1p(x); }
     DISCO_HD static double atan2_d(double y, double x) { return atan2(y, x); }
     DISCO_HD static double fabs_d(double x) { return fabs(x); }

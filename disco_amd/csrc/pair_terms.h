@@ -149,10 +149,22 @@ DISCO_HD float two_atanh_f(float z)
     return M::log_f((1.0f + z) * M::rcp_f(1.0f - z));
 }
 
+// 2 atanh(z) in double: for z < 1/4 the odd Taylor series through z^27
+// (first omitted term y^14/29 < 1e-18 relative, y = z^2 <= 1/16), else
+// log1p(2z / (1 - z)).
 template <class M>
 DISCO_HD double two_atanh_d(double z)
 {
-    return M::log1p_d(2.0 * z / (1.0 - z));
+    if (z < 0.25) {
+        const double y = z * z;
+        double p = 1.0 / 27;
+        p = p * y + 1.0 / 25; p = p * y + 1.0 / 23; p = p * y + 1.0 / 21;
+        p = p * y + 1.0 / 19; p = p * y + 1.0 / 17; p = p * y + 1.0 / 15;
+        p = p * y + 1.0 / 13; p = p * y + 1.0 / 11; p = p * y + 1.0 / 9;
+        p = p * y + 1.0 / 7;  p = p * y + 1.0 / 5;  p = p * y + 1.0 / 3;
+        return 2.0 * z + (2.0 * z * y) * p;
+    }
+    return M::log1p_d(2.0 * z * M::rcp_d(1.0 - z));
 }
 
 // ------------------------------------------------------------ line, FP32
@@ -357,13 +369,14 @@ DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __rest
                              double bx, double by, double bz, double& ox, double& oy, double& oz)
 {
     const double l1sq = qx * qx + qy * qy + qz * qz;
-    const double l1 = M::sqrt_d(l1sq);
-    const double rl1 = 1.0 / l1;
+    const double rl1 = M::rsqrt_d(l1sq);
+    const double l1 = l1sq * rl1;
 
     const SegD& last = s[nv - 1];
     double ax = last.bx - qx, ay = last.by - qy, az = last.bz - qz;
-    double RA = M::sqrt_d(ax * ax + ay * ay + az * az);
-    double rA = 1.0 / RA;
+    const double la2 = ax * ax + ay * ay + az * az;
+    double rA = M::rsqrt_d(la2);
+    double RA = la2 * rA;
     double P = l1 * RA + (l1sq - (qx * last.bx + qy * last.by + qz * last.bz));
 
     double X = 1.0, Y = 0.0;
@@ -371,8 +384,9 @@ DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __rest
     for (int k = 0; k < nv; k++) {
         const SegD& g = s[k];
         const double px = g.bx - qx, py = g.by - qy, pz = g.bz - qz;
-        const double RB = M::sqrt_d(px * px + py * py + pz * pz);
-        const double rB = 1.0 / RB;
+        const double lb2 = px * px + py * py + pz * pz;
+        const double rB = M::rsqrt_d(lb2);
+        const double RB = lb2 * rB;
 
         const double r1b = l1sq - (qx * g.bx + qy * g.by + qz * g.bz);
         const double ab = ax * px + ay * py + az * pz;
@@ -401,7 +415,7 @@ DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __rest
             kd = dx * dx + dy * dy + dz * dz;
             kn = sB * rB - sA * rA;
         }
-        const double rr = 1.0 / (RS * kd);
+        const double rr = M::rcp_d(RS * kd);
         const double J0 = two_atanh_d<M>(g.L * kd * rr);
         const double K0 = kn * RS * rr;
         const double K1 = rA - rB;
@@ -500,9 +514,8 @@ DISCO_HD void loopon_image_f64(double qx, double qy, double qz, const double4* _
         const double4 v = P[p];
         const double rx = qx - v.x, ry = qy - v.y, rz = qz - v.z;
         const double r2 = rx * rx + ry * ry + rz * rz;
-        const double r = M::sqrt_d(r2);
-        const bool keep = !(r < 1e-10);                 // Code/kroupa.hpp:59
-        const double ri = keep ? 1.0 / r : 1.0;
+        const bool keep = !(r2 < 1e-20);                // r < 1e-10, Code/kroupa.hpp:59
+        const double ri = M::rsqrt_d(keep ? r2 : 1.0);
         const double ri2 = ri * ri;
         const double ri3 = ri2 * ri;
         const double mr = mx * rx + my * ry + mz * rz;
