@@ -55,6 +55,11 @@ __host__ __device__ inline bf16 f2bf(float f)
     u += 0x7fffu + ((u >> 16) & 1u);
     return (bf16)(u >> 16);
 }
+// the same rounding (nearest even) by the hardware conversion, on the device
+__device__ inline bf16 f2bf_d(float f)
+{
+    return __builtin_bit_cast(bf16, (__bf16)f);
+}
 __host__ __device__ inline float bf2f(bf16 h)
 {
     const uint32_t u = (uint32_t)h << 16;
@@ -62,7 +67,28 @@ __host__ __device__ inline float bf2f(bf16 h)
     std::memcpy(&f, &u, 4);
     return f;
 }
-__device__ inline float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// logistic function: hardware exp and reciprocal (one rounding each)
+__device__ inline float sigmoidf_(float z)
+{
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+__device__ inline float siluf_(float z) { return z * sigmoidf_(z); }
+
+// Layer-1 pre-activation of neuron j: the loop part first, in feature order
+// 3..15, then the three offset features -- the order the fused apply uses with
+// the loop part precomputed per loop, so both paths round identically.
+__device__ inline float layer1_loop_part(const float* __restrict__ w, float b, const float* f)
+{
+    float z = b;
+#pragma unroll
+    for (int k = 3; k < kIn; k++) z = fmaf(f[k], w[k], z);
+    return z;
+}
+__device__ inline float layer1_offsets(float c, float w0, float w1, float w2, float d0, float d1,
+                                       float d2)
+{
+    return fmaf(d2, w2, fmaf(d1, w1, fmaf(d0, w0, c)));
+}
 
 // Features of one (atom, loop) pair; shared by the host feature entry point
 // and the device kernels, so training and evaluation see the same inputs.
@@ -101,11 +127,10 @@ __global__ void layer1_kernel(const float* __restrict__ X, const float* __restri
     const int r = (int)(t / kH), j = (int)(t % kH);
     const float* w = P + oW1 + j * kIn;
     const float* x = X + (long long)r * kIn;
-    float z = P[oB1 + j];
-#pragma unroll
-    for (int k = 0; k < kIn; k++) z += x[k] * w[k];
+    const float z = layer1_offsets(layer1_loop_part(w, P[oB1 + j], x), w[0], w[1], w[2], x[0],
+                                   x[1], x[2]);
     if (Z) Z[t] = z;
-    A[t] = f2bf(z * sigmoidf_(z));
+    A[t] = f2bf_d(siluf_(z));
 }
 
 // C (M x N) = A (M x K) . B (K x N) on MFMA, BF16 operands, FP32 accumulate.
@@ -146,7 +171,7 @@ __global__ __launch_bounds__(64) void gemm_bf16_kernel(
             C[ic * ldc + jc] = z;
         } else {
             if (C) C[ic * ldc + jc] = z;
-            Ab[ic * ldab + jc] = f2bf(z * sigmoidf_(z));
+            Ab[ic * ldab + jc] = f2bf_d(siluf_(z));
         }
     }
 }
@@ -282,6 +307,137 @@ __global__ void far_out_kernel(const float* __restrict__ Y, const disco_surrogat
         sx += s * y[0]; sy += s * y[1]; sz += s * y[2];
     }
     ux[a0 + t] += sx; uy[a0 + t] += sy; uz[a0 + t] += sz;
+}
+
+// ------------------------------------------------------------------ fused apply
+// Per loop: the layer-1 pre-activation with the loop's own features (n, b, R,
+// shape: constant for the loop) folded in, and the output scale gain |b|(R/L)^4.
+__global__ void loop_consts_kernel(const float* __restrict__ P,
+                                   const disco_surrogate_loop* __restrict__ loops, int nl,
+                                   double Lx, double Ly, double Lz, double gain,
+                                   float* __restrict__ cl, float* __restrict__ sl)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nl * kH) return;
+    const int l = t / kH, j = t % kH;
+    const double L[3] = {Lx, Ly, Lz};
+    float f[kIn];
+    pair_features(loops[l].c[0], loops[l].c[1], loops[l].c[2], loops[l], L, f);
+    cl[t] = layer1_loop_part(P + oW1 + j * kIn, P[oB1 + j], f);
+    if (j == 0) sl[l] = (float)(gain * pair_scale(loops[l], L));
+}
+
+// LDS rows of the BF16 weights are padded to 132 entries (264 B = 66 banks),
+// so the 32 rows an MFMA operand read touches fall on distinct bank pairs.
+constexpr int kLd = kH + 4;
+
+// One hidden layer of the fused apply: hb (the B operands of its 16 k-steps)
+// is replaced by the layer's output in the same layout.
+__device__ __forceinline__ void hidden_layer(const bf16* __restrict__ W,
+                                             const float* __restrict__ bias, int col, int h,
+                                             v4s (&hb)[16])
+{
+    v4s hn[16];
+#pragma unroll
+    for (int np = 0; np < 4; np += 2) {
+        v16f acc0, acc1;
+        for (int t = 0; t < 16; t++) { acc0[t] = 0.0f; acc1[t] = 0.0f; }
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            const v4s w0 = *reinterpret_cast<const v4s*>(W + (32 * np + col) * kLd + 8 * s + 4 * h);
+            const v4s w1 = *reinterpret_cast<const v4s*>(W + (32 * np + 32 + col) * kLd + 8 * s + 4 * h);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w0, hb[s], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w1, hb[s], acc1, 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            const int n0 = 32 * np + 8 * (t >> 2) + 4 * h + (t & 3);
+            hn[4 * np + (t >> 2)][t & 3] = (short)f2bf_d(siluf_(acc0[t] + bias[n0]));
+            hn[4 * np + 4 + (t >> 2)][t & 3] = (short)f2bf_d(siluf_(acc1[t] + bias[n0 + 32]));
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 16; s++) hb[s] = hn[s];
+}
+
+// u[a] += sum over loops of scale_l * MLP(features(a, l)), the whole network in
+// one kernel.  A wave holds 32 atoms (one per MFMA column) and walks the loops;
+// each layer is computed transposed, out^T = W . h^T, with the weights as the
+// MFMA A operand (from LDS) and the activations as B: the accumulator of one
+// 32-neuron tile (lane: its column; register t: neuron 8(t/4) + 4(lane/32) +
+// t%4) is exactly the B operand of the next layer's four k-steps, so
+// activations never leave registers.  Layer 1 (three offset features; the
+// loop's features are in cl) runs on the VALU in FP32.
+__global__ __launch_bounds__(256) void apply_fused_kernel(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    long long a0, int na, const disco_surrogate_loop* __restrict__ loops, int nl, double Lx,
+    double Ly, double Lz, const float* __restrict__ P, const bf16* __restrict__ Wb,
+    const float* __restrict__ cl, const float* __restrict__ sl, double* __restrict__ ux,
+    double* __restrict__ uy, double* __restrict__ uz)
+{
+    __shared__ bf16 sW2[kH * kLd], sW3[kH * kLd], sW4[kOut * kLd];
+    __shared__ float4 sW1[kH];
+    __shared__ float sb2[kH], sb3[kH], sb4[kOut];
+    for (int i = threadIdx.x; i < kH * kH; i += 256) {
+        const int r = i / kH, c = i % kH;
+        sW2[r * kLd + c] = Wb[i];
+        sW3[r * kLd + c] = Wb[kH * kH + i];
+        if (r < kOut) sW4[r * kLd + c] = Wb[2 * kH * kH + i];
+    }
+    for (int i = threadIdx.x; i < kH; i += 256) {
+        const float* w = P + oW1 + i * kIn;
+        sW1[i] = make_float4(w[0], w[1], w[2], 0.0f);
+        sb2[i] = P[oB2 + i];
+        sb3[i] = P[oB3 + i];
+        if (i < kOut) sb4[i] = P[oB4 + i];
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, col = lane & 31, h = lane >> 5;
+    const int ia = blockIdx.x * 128 + (threadIdx.x >> 6) * 32 + col;   // atom of this column
+    const bool valid = ia < na;
+    const long long a = a0 + (valid ? ia : na - 1);
+    const double px = x[a], py = y[a], pz = z[a];
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+    for (int l = 0; l < nl; l++) {
+        // the weights stay in LDS: reloaded per loop, not hoisted into registers
+        asm volatile("" ::: "memory");
+        const disco_surrogate_loop& lp = loops[l];
+        const float d0 = (float)((px - lp.c[0]) / Lx);
+        const float d1 = (float)((py - lp.c[1]) / Ly);
+        const float d2 = (float)((pz - lp.c[2]) / Lz);
+        const float* c = cl + (long long)l * kH;
+        // layer 1: this lane's 64 neurons 8s + 4h + i, as the B operands of layer 2
+        v4s hb[16];
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            const float4 cc = *reinterpret_cast<const float4*>(c + 8 * s + 4 * h);
+            const float cv[4] = {cc.x, cc.y, cc.z, cc.w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float4 w = sW1[8 * s + 4 * h + i];
+                hb[s][i] = (short)f2bf_d(siluf_(layer1_offsets(cv[i], w.x, w.y, w.z, d0, d1, d2)));
+            }
+        }
+        // layers 2 and 3: 128 x 128, 32-neuron tiles two at a time, 16 k-steps
+        hidden_layer(sW2, sb2, col, h, hb);
+        hidden_layer(sW3, sb3, col, h, hb);
+        // layer 4: 32 outputs (3 used), 16 k-steps
+        v16f o4;
+        for (int t = 0; t < 16; t++) o4[t] = 0.0f;
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            const v4s wa = *reinterpret_cast<const v4s*>(sW4 + col * kLd + 8 * s + 4 * h);
+            o4 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(wa, hb[s], o4, 0, 0, 0);
+        }
+        const float sc = sl[l];
+        ox = fmaf(sc, o4[0] + sb4[0], ox);
+        oy = fmaf(sc, o4[1] + sb4[1], oy);
+        oz = fmaf(sc, o4[2] + sb4[2], oz);
+    }
+    if (h == 0 && valid) {             // outputs 0..2 sit in lanes 0..31, registers 0..2
+        ux[a] += ox; uy[a] += oy; uz[a] += oz;
+    }
 }
 
 __global__ void zero_scalar_kernel(float* __restrict__ p) { if (threadIdx.x == 0) *p = 0.0f; }
@@ -629,28 +785,30 @@ int disco_hip_surrogate_apply(disco_surrogate* s, int natoms, const double* x, c
     DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
     hipStream_t st = (hipStream_t)stream;
     if (chunk_rows <= 0) chunk_rows = 1 << 22;
-    const int apc = std::max(1, chunk_rows / nloops);     // atoms per chunk
-    const int rows = apc * nloops;
-    int rc = ensure_acts(s, rows);
-    if (rc != DISCO_OK) return rc;
-    disco_surrogate_loop* dl = nullptr;
-    DISCO_HIP_TRY(ctx, hipMalloc(&dl, (size_t)nloops * sizeof(disco_surrogate_loop)));
-    if (hipMemcpy(dl, loops, (size_t)nloops * sizeof(disco_surrogate_loop), hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipFree(dl);
+    const int apc = std::max(1, chunk_rows / nloops);     // atoms per launch
+    // loop table, then per loop the layer-1 part of its own features and its scale
+    char* buf = nullptr;
+    const size_t lb = (size_t)nloops * sizeof(disco_surrogate_loop);
+    const size_t cb = (size_t)nloops * kH * sizeof(float);
+    DISCO_HIP_TRY(ctx, hipMalloc(&buf, lb + cb + (size_t)nloops * sizeof(float) + 256));
+    disco_surrogate_loop* dl = reinterpret_cast<disco_surrogate_loop*>(buf);
+    float* cl = reinterpret_cast<float*>(buf + ((lb + 255) / 256) * 256);
+    float* sl = cl + (size_t)nloops * kH;
+    if (hipMemcpy(dl, loops, lb, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(buf);
         return fail(ctx, DISCO_ERR_HIP, "loop upload");
     }
+    hipLaunchKernelGGL(loop_consts_kernel, dim3(blocks((long long)nloops * kH, 256)), dim3(256),
+                       0, st, s->P, dl, nloops, L[0], L[1], L[2], s->gain, cl, sl);
     for (long long a0 = 0; a0 < natoms; a0 += apc) {
         const int na = (int)std::min<long long>(apc, natoms - a0);
-        const int M = na * nloops;
-        hipLaunchKernelGGL(pair_features_kernel, dim3(blocks(M, 256)), dim3(256), 0, st, x, y, z,
-                           a0, dl, nloops, L[0], L[1], L[2], s->X, M);
-        forward(s, M, false, st);
-        hipLaunchKernelGGL(far_out_kernel, dim3(blocks(na, 256)), dim3(256), 0, st, s->Y, dl,
-                           nloops, L[0], L[1], L[2], s->gain, a0, na, ux, uy, uz);
+        hipLaunchKernelGGL(apply_fused_kernel, dim3(blocks(na, 128)), dim3(256), 0, st, x, y, z,
+                           a0, na, dl, nloops, L[0], L[1], L[2], s->P, s->Wb, cl, sl, ux, uy,
+                           uz);
     }
     const hipError_t e = hipGetLastError();
     (void)hipStreamSynchronize(st);
-    (void)hipFree(dl);
+    (void)hipFree(buf);
     if (e != hipSuccess) return fail(ctx, DISCO_ERR_HIP, std::string("apply: ") + hipGetErrorString(e));
     return DISCO_OK;
 }

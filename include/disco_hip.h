@@ -248,9 +248,10 @@ int disco_hip_surrogate_loss_grad(disco_surrogate *sur, const float *feats,
 int disco_hip_surrogate_train(disco_surrogate *sur, const float *feats, const float *targets,
                               int n, int epochs, int batch, float lr, unsigned seed,
                               float *final_loss);
-/* u += sum over loops of gain * scale * network(features) for device atoms; rows of
- * (atom, loop) pairs are processed chunk_rows at a time (<= 0: 4M).
- * Synchronous on `stream`. */
+/* u += sum over loops of gain * scale * network(features) for device atoms.  The
+ * whole network runs in one fused kernel (weights in LDS, activations in
+ * registers); atoms are launched chunk_rows / nloops at a time (chunk_rows <= 0:
+ * 4M rows).  Synchronous on `stream`. */
 int disco_hip_surrogate_apply(disco_surrogate *sur, int natoms, const double *d_x,
                               const double *d_y, const double *d_z, double *d_ux, double *d_uy,
                               double *d_uz, const disco_surrogate_loop *loops, int nloops,
