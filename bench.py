@@ -84,10 +84,12 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "copy"],
+    ap.add_argument("--workload", default="c2",
+                    choices=["c2", "c3", "c4", "c5", "copy", "lattice"],
                     help="c2 (default): the 4M-atom line-integral sample; c3: loopons; "
                          "c4: 256 batched samples; c5: the far-shell surrogate; copy: a plain "
-                         "256 MiB device copy (x3), the size reference for HBM byte counters")
+                         "256 MiB device copy (x3), the size reference for HBM byte counters; "
+                         "lattice: the 4M-atom lattice generated on the device")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): atoms shard "

@@ -169,6 +169,9 @@ def lib():
         L.disco_hip_batch_evals.argtypes = [_vp]
         L.disco_hip_batch_evals.restype = _d
         L.disco_hip_batch_free.argtypes = [_vp]
+        L.disco_hip_lattice_atoms.argtypes = [_i, _i, _i, _i]
+        L.disco_hip_lattice_atoms.restype = ctypes.c_longlong
+        L.disco_hip_lattice.argtypes = [_vp, _i, _i, _i, _i, _d, _vp, _vp, _vp, _vp]
         _lib = L
     return _lib
 
@@ -179,6 +182,35 @@ def _check(code, ctx=None):
         detail = L.disco_hip_last_error(ctx).decode() if ctx else ""
         raise DiscoError(code, (L.disco_hip_strerror(code).decode() +
                                 (f" ({detail})" if detail else "")))
+
+
+# ------------------------------------------------------------------ lattice
+LATTICE = {"BCC": 0, "FCC": 1}
+
+
+def lattice_atoms(kind: str, nx: int, ny: int, nz: int) -> int:
+    n = lib().disco_hip_lattice_atoms(LATTICE[kind], nx, ny, nz)
+    if n < 0:
+        raise DiscoError(1, "invalid lattice arguments")
+    return int(n)
+
+
+def lattice_torch(ctx, kind: str, nx: int, ny: int, nz: int, a0: float, out=None):
+    """Perfect lattice generated on the device (generateLattice, Code/lattice.hpp:35-51):
+    returns device tensors x, y, z (FP64), or fills `out` (three tensors of
+    lattice_atoms() elements each)."""
+    import torch
+    n = lattice_atoms(kind, nx, ny, nz)
+    if out is None:
+        out = [torch.empty(n, dtype=torch.float64, device=f"cuda:{ctx.device}") for _ in range(3)]
+    for t in out:
+        if t.numel() < n or t.dtype != torch.float64 or not t.is_contiguous():
+            raise DiscoError(1, "output tensors must be contiguous FP64 with room for every site")
+    stream = torch.cuda.current_stream().cuda_stream
+    ctx.check(lib().disco_hip_lattice(ctx._h, LATTICE[kind], nx, ny, nz, float(a0),
+                                      out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(),
+                                      stream))
+    return out
 
 
 # ------------------------------------------------------------------ devices
@@ -215,6 +247,7 @@ class Context:
 
     def __init__(self, device: int = 0):
         self._h = _vp()
+        self.device = device
         _check(lib().disco_hip_open(device, ctypes.byref(self._h)))
 
     @property

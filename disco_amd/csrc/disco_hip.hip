@@ -401,6 +401,34 @@ __global__ __launch_bounds__(kBlock) void zero3_kernel(double* __restrict__ a,
     if (i < n) { a[i] = 0.0; b[i] = 0.0; c[i] = 0.0; }
 }
 
+// ------------------------------------------------------------------ lattice
+// Sites of a cubic lattice of nx x ny x nz cells centred on the origin, in the
+// order of Code/lattice.hpp:44-48 (cells i, j, k with k fastest, then the basis
+// sites), each coordinate (i a0 + f a0) - h rounded after every operation (no
+// contraction), as the reference's host loop evaluates it.  Grid: (row tiles,
+// ny, nx); a row is the nz * NB sites of one (i, j).
+template <int NB>
+__global__ __launch_bounds__(kBlock) void lattice_kernel(int ny, int nz, double a0, double hx,
+                                                         double hy, double hz,
+                                                         double* __restrict__ x,
+                                                         double* __restrict__ y,
+                                                         double* __restrict__ z)
+{
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * kBlock + threadIdx.x;       // site within the row
+    if (t >= nz * NB) return;
+    const int i = blockIdx.z, j = blockIdx.y, k = t / NB, b = t - k * NB;
+    // basis (Code/lattice.hpp:38-39): BCC {0, (1/2,1/2,1/2)}; FCC {0, (1/2,1/2,0),
+    // (1/2,0,1/2), (0,1/2,1/2)}
+    const double fx = NB == 2 ? (b ? 0.5 : 0.0) : (b == 1 || b == 2 ? 0.5 : 0.0);
+    const double fy = NB == 2 ? (b ? 0.5 : 0.0) : (b == 1 || b == 3 ? 0.5 : 0.0);
+    const double fz = NB == 2 ? (b ? 0.5 : 0.0) : (b == 2 || b == 3 ? 0.5 : 0.0);
+    const long long o = ((long long)i * ny + j) * ((long long)nz * NB) + t;
+    x[o] = ((double)i * a0 + fx * a0) - hx;
+    y[o] = ((double)j * a0 + fy * a0) - hy;
+    z[o] = ((double)k * a0 + fz * a0) - hz;
+}
+
 // ------------------------------------------------------------------ batched samples
 // One sample of a batch: its atoms [a0, a0 + n) of the concatenated arrays,
 // its loops [kofs[0], kofs[4]) of the concatenated loop table, those of kind k
@@ -909,6 +937,41 @@ int disco_hip_device_info(int device, disco_device_info* info)
 }
 
 int disco_hip_line_fp64(void) { return 1; }
+
+long long disco_hip_lattice_atoms(int lattice, int nx, int ny, int nz)
+{
+    if ((lattice != DISCO_LATTICE_BCC && lattice != DISCO_LATTICE_FCC) || nx < 0 || ny < 0 ||
+        nz < 0)
+        return -1;
+    return (long long)(lattice == DISCO_LATTICE_FCC ? 4 : 2) * nx * ny * nz;
+}
+
+int disco_hip_lattice(disco_ctx* ctx, int lattice, int nx, int ny, int nz, double a0,
+                      double* d_x, double* d_y, double* d_z, void* stream)
+{
+    if (!ctx) return DISCO_ERR_ARG;
+    if (lattice != DISCO_LATTICE_BCC && lattice != DISCO_LATTICE_FCC)
+        return fail(ctx, DISCO_ERR_ARG, "unknown lattice");
+    if (nx < 0 || ny < 0 || nz < 0) return fail(ctx, DISCO_ERR_ARG, "negative cell count");
+    if (nx > 65535 || ny > 65535 || nz > 65535)
+        return fail(ctx, DISCO_ERR_TOO_LARGE, "more than 65535 cells along an axis");
+    if (!std::isfinite(a0)) return fail(ctx, DISCO_ERR_ARG, "non-finite lattice constant");
+    if (nx == 0 || ny == 0 || nz == 0) return DISCO_OK;
+    if (!d_x || !d_y || !d_z) return fail(ctx, DISCO_ERR_ARG, "null device array");
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const int nb = lattice == DISCO_LATTICE_FCC ? 4 : 2;
+    const double hx = nx * a0 / 2., hy = ny * a0 / 2., hz = nz * a0 / 2.;   // Code/lattice.hpp:43
+    const dim3 grid((unsigned)((nz * nb + kBlock - 1) / kBlock), (unsigned)ny, (unsigned)nx);
+    hipStream_t s = (hipStream_t)stream;
+    if (nb == 4)
+        hipLaunchKernelGGL(lattice_kernel<4>, grid, dim3(kBlock), 0, s, ny, nz, a0, hx, hy, hz,
+                           d_x, d_y, d_z);
+    else
+        hipLaunchKernelGGL(lattice_kernel<2>, grid, dim3(kBlock), 0, s, ny, nz, a0, hx, hy, hz,
+                           d_x, d_y, d_z);
+    DISCO_HIP_TRY(ctx, hipGetLastError());
+    return DISCO_OK;
+}
 
 int disco_hip_open(int device, disco_ctx** out)
 {

@@ -257,6 +257,19 @@ int disco_hip_surrogate_apply(disco_surrogate *sur, int natoms, const double *d_
                               double *d_uz, const disco_surrogate_loop *loops, int nloops,
                               const double *L, int chunk_rows, void *stream);
 
+/* ---------------------------------------------------------------- lattice
+ * Perfect cubic lattice written straight into device memory, in the reference's
+ * atom order and bit for bit its coordinates: replaces generateLattice
+ * (Code/lattice.hpp:35-51; called at Code/main.cpp:576-584) and the host-to-
+ * device copy of the positions.  lattice: DISCO_LATTICE_BCC / _FCC; the
+ * caller allocates disco_hip_lattice_atoms() doubles per array; 0 <= nx, ny,
+ * nz <= 65535.  Asynchronous on `stream`. */
+#define DISCO_LATTICE_BCC 0
+#define DISCO_LATTICE_FCC 1
+long long disco_hip_lattice_atoms(int lattice, int nx, int ny, int nz);
+int disco_hip_lattice(disco_ctx *ctx, int lattice, int nx, int ny, int nz, double a0,
+                      double *d_x, double *d_y, double *d_z, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,7 +17,7 @@ HEADER = os.path.join(ROOT, "include", "disco_hip.h")
 
 def _declared():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|double|const char \*)\s*(disco_hip_\w+)\s*\(",
+    return sorted(set(re.findall(r"^\s*(?:int|double|long long|const char \*)\s*(disco_hip_\w+)\s*\(",
                                  text, flags=re.M)))
 
 
@@ -64,6 +64,12 @@ def test_error_codes_without_a_device():
                                 1.0, 1.0, 1.0, 1, 0, None)
     assert rc == 1
     assert lib.disco_hip_plan_run(None, 0, None, None, None, None, None, None, 0, None) == 1
+    # lattice sizes are host arithmetic; a null context is an argument error
+    assert B.lattice_atoms("FCC", 100, 100, 100) == 4_000_000
+    assert B.lattice_atoms("BCC", 40, 41, 42) == 2 * 40 * 41 * 42
+    assert lib.disco_hip_lattice_atoms(7, 1, 1, 1) == -1
+    assert lib.disco_hip_lattice_atoms(0, -1, 1, 1) == -1
+    assert lib.disco_hip_lattice(None, 0, 1, 1, 1, 1.0, None, None, None, None) == 1
     n = ctypes.c_int(-1)
     assert lib.disco_hip_device_count(ctypes.byref(n)) == 0 and n.value >= 0
     if n.value == 0:
