@@ -429,6 +429,59 @@ __global__ __launch_bounds__(kBlock) void lattice_kernel(int ny, int nz, double 
     z[o] = ((double)k * a0 + fz * a0) - hz;
 }
 
+// ------------------------------------------------------------------ grid field
+// 4-node cubic Lagrange weights at fractional position f in [0, 1) of the
+// interval between nodes 1 and 2 of the stencil {0, 1, 2, 3}.
+__device__ __forceinline__ void cubic_weights(double f, double w[4])
+{
+    const double fm1 = f - 1.0, fm2 = f - 2.0, fp1 = f + 1.0;
+    w[0] = -f * fm1 * fm2 * (1.0 / 6.0);
+    w[1] = fp1 * fm1 * fm2 * 0.5;
+    w[2] = -fp1 * f * fm2 * 0.5;
+    w[3] = fp1 * f * fm1 * (1.0 / 6.0);
+}
+
+__device__ __forceinline__ int stencil_start(double t, int nn, double& f)
+{
+    double c = floor(t);
+    c = fmin(fmax(c, 1.0), (double)(nn - 3));     // nodes c-1 .. c+2 inside [0, nn)
+    f = t - c;
+    return (int)c - 1;
+}
+
+__global__ __launch_bounds__(kBlock) void grid_interp_kernel(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
+    const double* __restrict__ gx, const double* __restrict__ gy, const double* __restrict__ gz,
+    int nx, int ny, int nz, double ox, double oy, double oz, double ih, int accumulate)
+{
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= n) return;
+    double fx, fy, fz, wx[4], wy[4], wz[4];
+    const int i0 = stencil_start((x[a] - ox) * ih, nx, fx);
+    const int j0 = stencil_start((y[a] - oy) * ih, ny, fy);
+    const int k0 = stencil_start((z[a] - oz) * ih, nz, fz);
+    cubic_weights(fx, wx);
+    cubic_weights(fy, wy);
+    cubic_weights(fz, wz);
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int k = 0; k < 4; k++)
+        for (int j = 0; j < 4; j++) {
+            const double wjk = wy[j] * wz[k];
+            const long long row = ((long long)(k0 + k) * ny + (j0 + j)) * nx + i0;
+            double rx = 0.0, ry = 0.0, rz = 0.0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                rx += wx[i] * gx[row + i];
+                ry += wx[i] * gy[row + i];
+                rz += wx[i] * gz[row + i];
+            }
+            sx += wjk * rx; sy += wjk * ry; sz += wjk * rz;
+        }
+    if (accumulate) { sx += ux[a]; sy += uy[a]; sz += uz[a]; }
+    ux[a] = sx; uy[a] = sy; uz[a] = sz;
+}
+
 // ------------------------------------------------------------------ batched samples
 // One sample of a batch: its atoms [a0, a0 + n) of the concatenated arrays,
 // its loops [kofs[0], kofs[4]) of the concatenated loop table, those of kind k
@@ -937,6 +990,29 @@ int disco_hip_device_info(int device, disco_device_info* info)
 }
 
 int disco_hip_line_fp64(void) { return 1; }
+
+int disco_hip_grid_interp(disco_ctx* ctx, int n, const double* d_x, const double* d_y,
+                          const double* d_z, double* d_ux, double* d_uy, double* d_uz,
+                          const double* d_gx, const double* d_gy, const double* d_gz, int nx,
+                          int ny, int nz, const double* origin, double h, int accumulate,
+                          void* stream)
+{
+    if (!ctx) return DISCO_ERR_ARG;
+    if (n < 0) return fail(ctx, DISCO_ERR_ARG, "negative size");
+    if (nx < 4 || ny < 4 || nz < 4) return fail(ctx, DISCO_ERR_ARG, "a grid needs >= 4 nodes per axis");
+    if (!origin || !(h > 0.0) || !std::isfinite(h))
+        return fail(ctx, DISCO_ERR_ARG, "bad grid origin or spacing");
+    if (n == 0) return DISCO_OK;
+    if (!d_x || !d_y || !d_z || !d_ux || !d_uy || !d_uz || !d_gx || !d_gy || !d_gz)
+        return fail(ctx, DISCO_ERR_ARG, "null device array");
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(grid_interp_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, (hipStream_t)stream, d_x, d_y, d_z, d_ux, d_uy, d_uz, n,
+                       d_gx, d_gy, d_gz, nx, ny, nz, origin[0], origin[1], origin[2], 1.0 / h,
+                       accumulate);
+    DISCO_HIP_TRY(ctx, hipGetLastError());
+    return DISCO_OK;
+}
 
 long long disco_hip_lattice_atoms(int lattice, int nx, int ny, int nz)
 {

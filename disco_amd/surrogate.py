@@ -303,15 +303,43 @@ def exact_far_shell_torch(ctx, x, y, z, loops, box, img, prec=B.PREC_FP32C):
     return u
 
 
+def grid_nodes(x, y, z, h):
+    """Uniform grid of spacing h covering the device atoms with one node of margin
+    on each side (the 4-node stencils of the interpolation stay inside): returns
+    the origin, the node counts (nx, ny, nz) and the node coordinates (three
+    device tensors of nz * ny * nx, x fastest)."""
+    import torch
+    lo = [float(t.min()) for t in (x, y, z)]
+    hi = [float(t.max()) for t in (x, y, z)]
+    dims = [int(math.ceil((b - a) / h)) + 4 for a, b in zip(lo, hi)]
+    origin = [a - h for a in lo]
+    axes = [origin[k] + h * torch.arange(dims[k], dtype=torch.float64, device=x.device)
+            for k in range(3)]
+    gz, gy, gx = torch.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    return origin, dims, (gx.reshape(-1).contiguous(), gy.reshape(-1).contiguous(),
+                          gz.reshape(-1).contiguous())
+
+
 def base_far_shell_torch(ctx, x, y, z, loops, box, img, kind="rule7", u=None,
-                         accumulate=False, prec=B.PREC_FP32C):
-    """Far shell of the loops' loopon rule (`kind`), summed by the loopon kernels."""
+                         accumulate=False, prec=B.PREC_FP32C, grid_h=None):
+    """Far shell of the loops' loopon rule (`kind`), summed by the loopon kernels:
+    at every atom, or (grid_h set) at the nodes of a grid of that spacing over
+    the atoms and interpolated to them -- the far shell is smooth on the scale of
+    the box, so a grid of L/32 carries it to ~1e-9 A."""
     import torch
     if u is None:
         u = [torch.zeros_like(x) for _ in range(3)]
     plan = B.Plan(ctx, base_loopons(loops, kind), box, img, prec, kind="loopon")
     set_inner(plan, 1)
-    plan.run_torch(x, y, z, *u, accumulate=accumulate)
+    if grid_h is None:
+        plan.run_torch(x, y, z, *u, accumulate=accumulate)
+    else:
+        origin, dims, nodes = grid_nodes(x, y, z, grid_h)
+        gu = [torch.zeros_like(nodes[0]) for _ in range(3)]
+        plan.run_torch(*nodes, *gu)
+        shape = (dims[2], dims[1], dims[0])
+        B.grid_interp_torch(ctx, x, y, z, *u, [t.view(shape) for t in gu], origin, grid_h,
+                            accumulate=accumulate)
     plan.free()
     return u
 
@@ -389,7 +417,7 @@ class FarShellReport:
 
 
 def far_shell(ctx, sur: Surrogate, x, y, z, u, loops, infos, box, img, tol=1e-5,
-              n_valid=4096, seed=3, exact_if_failed=True, baseline="rule7"):
+              n_valid=4096, seed=3, exact_if_failed=True, baseline="rule7", grid_h=None):
     """u += far shell of the loops at the device atoms (x, y, z): the surrogate
     (baseline rule + MLP) when its error on n_valid random atoms is within tol
     of the exact far shell, else the exact far shell.  Returns a FarShellReport."""
@@ -400,14 +428,22 @@ def far_shell(ctx, sur: Surrogate, x, y, z, u, loops, infos, box, img, tol=1e-5,
     xv, yv, zv = x[idx].contiguous(), y[idx].contiguous(), z[idx].contiguous()
     # judge: exact far shell on the validation atoms
     ex = exact_far_shell_torch(ctx, xv, yv, zv, loops, box, img)
-    dv = base_far_shell_torch(ctx, xv, yv, zv, loops, box, img, baseline)
+    # the base as the full run computes it (on the grid of the full atom set)
+    dv = [torch.zeros_like(xv) for _ in range(3)]
+    if grid_h is None:
+        base_far_shell_torch(ctx, xv, yv, zv, loops, box, img, baseline, u=dv)
+    else:
+        full = base_far_shell_torch(ctx, x, y, z, loops, box, img, baseline, grid_h=grid_h)
+        dv = [t[idx].contiguous() for t in full]
+        del full
     dip_only = max(float((e - d).abs().max()) for e, d in zip(ex, dv))
     sur.apply_torch(xv, yv, zv, *dv, infos, box)
     torch.cuda.synchronize()
     err = max(float((e - d).abs().max()) for e, d in zip(ex, dv))
     if err <= tol:
         t0 = time.perf_counter()
-        base_far_shell_torch(ctx, x, y, z, loops, box, img, baseline, u=u, accumulate=True)
+        base_far_shell_torch(ctx, x, y, z, loops, box, img, baseline, u=u, accumulate=True,
+                             grid_h=grid_h)
         sur.apply_torch(x, y, z, *u, infos, box)
         torch.cuda.synchronize()
         return FarShellReport(True, err, dip_only, baseline, idx.numel(),

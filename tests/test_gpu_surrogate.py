@@ -157,3 +157,44 @@ def test_far_shell_falls_back_to_exact_when_untrained(hip_ctx, sur):
     assert not rep.used_surrogate and rep.max_err > 1e-9
     ex = S.exact_far_shell_torch(hip_ctx, x, y, z, c2.loops, c2.box, 2)
     assert max(float((a - b).abs().max()) for a, b in zip(u, ex)) == 0.0
+
+
+def test_grid_interp_reproduces_cubics(hip_ctx):
+    """Tensor-product 4-node cubic Lagrange is exact for degree <= 3 per axis;
+    accumulate adds, overwrite writes."""
+    import torch
+    rng = np.random.default_rng(7)
+    h, origin, dims = 1.7, (-3.0, 2.0, 0.5), (9, 7, 11)
+    f = lambda X, Y, Z: ((1 + 2 * X - 0.3 * X ** 2 + 0.05 * X ** 3) * (3 - Y + 0.2 * Y * Y)
+                         * (0.01 * Z ** 3 - 2 * Z + 1))
+    ax = [origin[k] + h * np.arange(dims[k]) for k in range(3)]
+    Zg, Yg, Xg = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    g = [torch.from_numpy(np.ascontiguousarray(c * f(Xg, Yg, Zg))).cuda() for c in (1.0, -2.0, 0.5)]
+    P = np.stack([rng.uniform(origin[k] + h, origin[k] + h * (dims[k] - 2), 500) for k in range(3)], 1)
+    x, y, z = (torch.from_numpy(np.ascontiguousarray(P[:, k])).cuda() for k in range(3))
+    u = [torch.full_like(x, 5.0) for _ in range(3)]
+    B.grid_interp_torch(hip_ctx, x, y, z, *u, g, origin, h, accumulate=False)
+    want = f(P[:, 0], P[:, 1], P[:, 2])
+    for c, t in zip((1.0, -2.0, 0.5), u):
+        assert np.abs(t.cpu().numpy() - c * want).max() <= 1e-10 * np.abs(want).max()
+    B.grid_interp_torch(hip_ctx, x, y, z, *u, g, origin, h, accumulate=True)
+    assert np.abs(u[0].cpu().numpy() - 2 * want).max() <= 1e-10 * np.abs(want).max()
+
+
+def test_grid_far_shell_matches_the_direct_sum(hip_ctx):
+    """The rule's far shell (images 2..4) tabulated on an L/32 grid over the atoms
+    and interpolated equals the direct loopon sum at every atom."""
+    import torch
+    c2 = G.config_c2(nloops=6)
+    rng = np.random.default_rng(4)
+    idx = np.sort(rng.choice(c2.n, 3000, replace=False))
+    x, y, z = (torch.from_numpy(np.ascontiguousarray(a[idx])).cuda() for a in (c2.x, c2.y, c2.z))
+    for prec, tol in ((B.PREC_FP64, 1e-8), (B.PREC_FP32C, 1e-6)):
+        direct = S.base_far_shell_torch(hip_ctx, x, y, z, c2.loops, c2.box, 4, "rule7",
+                                        prec=prec)
+        grid = S.base_far_shell_torch(hip_ctx, x, y, z, c2.loops, c2.box, 4, "rule7",
+                                      prec=prec, grid_h=c2.box[0] / 32)
+        d = max(float((a - b).abs().max()) for a, b in zip(direct, grid))
+        m = max(float(a.abs().max()) for a in direct)
+        print(f"grid vs direct far shell, prec {prec}: {d:.3e} A (field up to {m:.3e} A)")
+        assert d < tol
