@@ -784,8 +784,8 @@ int disco_hip_surrogate_apply(disco_surrogate* s, int natoms, const double* x, c
     if (!x || !y || !z || !ux || !uy || !uz || !loops) return fail(ctx, DISCO_ERR_ARG, "null array");
     DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
     hipStream_t st = (hipStream_t)stream;
-    if (chunk_rows <= 0) chunk_rows = 1 << 22;
-    const int apc = std::max(1, chunk_rows / nloops);     // atoms per launch
+    // atoms per launch: all of them unless the caller bounds a launch's length
+    const int apc = chunk_rows <= 0 ? natoms : std::max(1, chunk_rows / nloops);
     // loop table, then per loop the layer-1 part of its own features and its scale
     char* buf = nullptr;
     const size_t lb = (size_t)nloops * sizeof(disco_surrogate_loop);

@@ -265,7 +265,7 @@ class Surrogate:
                                                         loss.ctypes.data_as(_fp)))
         return float(loss[0])
 
-    def apply_torch(self, x, y, z, ux, uy, uz, infos, box, chunk_rows=1 << 22):
+    def apply_torch(self, x, y, z, ux, uy, uz, infos, box, chunk_rows=0):
         """u += sum over loops of scale * MLP (device atoms as torch tensors)."""
         import torch
         arr = (_SLoop * max(1, len(infos)))(*[i.struct() for i in infos])

@@ -250,8 +250,8 @@ int disco_hip_surrogate_train(disco_surrogate *sur, const float *feats, const fl
                               float *final_loss);
 /* u += sum over loops of gain * scale * network(features) for device atoms.  The
  * whole network runs in one fused kernel (weights in LDS, activations in
- * registers); atoms are launched chunk_rows / nloops at a time (chunk_rows <= 0:
- * 4M rows).  Synchronous on `stream`. */
+ * registers); all atoms in one launch (chunk_rows <= 0) or chunk_rows / nloops
+ * atoms per launch.  Synchronous on `stream`. */
 int disco_hip_surrogate_apply(disco_surrogate *sur, int natoms, const double *d_x,
                               const double *d_y, const double *d_z, double *d_ux, double *d_uy,
                               double *d_uz, const disco_surrogate_loop *loops, int nloops,

@@ -183,13 +183,15 @@ def test_grid_interp_reproduces_cubics(hip_ctx):
 
 def test_grid_far_shell_matches_the_direct_sum(hip_ctx):
     """The rule's far shell (images 2..4) tabulated on an L/32 grid over the atoms
-    and interpolated equals the direct loopon sum at every atom."""
+    and interpolated equals the direct loopon sum at every atom (to 2e-7 A,
+    50x below the 1e-5 A bar)."""
     import torch
     c2 = G.config_c2(nloops=6)
     rng = np.random.default_rng(4)
     idx = np.sort(rng.choice(c2.n, 3000, replace=False))
     x, y, z = (torch.from_numpy(np.ascontiguousarray(a[idx])).cuda() for a in (c2.x, c2.y, c2.z))
-    for prec, tol in ((B.PREC_FP64, 1e-8), (B.PREC_FP32C, 1e-6)):
+    # the interpolation error of an L/32 grid: ~6e-6 of the far shell's ~1e-2 A
+    for prec, tol in ((B.PREC_FP64, 2e-7), (B.PREC_FP32C, 1e-6)):
         direct = S.base_far_shell_torch(hip_ctx, x, y, z, c2.loops, c2.box, 4, "rule7",
                                         prec=prec)
         grid = S.base_far_shell_torch(hip_ctx, x, y, z, c2.loops, c2.box, 4, "rule7",
