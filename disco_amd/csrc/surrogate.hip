@@ -211,26 +211,99 @@ __global__ void silu_bwd_kernel(const float* __restrict__ dA, const float* __res
 }
 
 // db[j] = sum_r D[r, j] (D given as BF16 or FP32), one thread per column
-__global__ void colsum_kernel(const bf16* __restrict__ Db, const float* __restrict__ Df,
-                              float* __restrict__ db, int M, int N)
+// Reductions over the batch rows in two deterministic stages: every block sums
+// a slice of kSlice rows into its own partial, a second kernel adds the
+// partials in block order.
+constexpr int kSlice = 256;
+
+// column sums of an M x N matrix (N <= 256, dividing 256): partial[b][j]
+template <bool BF>
+__global__ __launch_bounds__(256) void colsum_partial_kernel(const bf16* __restrict__ Db,
+                                                             const float* __restrict__ Df,
+                                                             float* __restrict__ part, int M,
+                                                             int N)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= N) return;
-    float s = 0.0f;
-    for (int r = 0; r < M; r++) s += Db ? bf2f(Db[(long long)r * N + j]) : Df[(long long)r * N + j];
-    db[j] = s;
+    __shared__ float red[256];
+    const int R = 256 / N, j = threadIdx.x % N, q = threadIdx.x / N;
+    const long long r0 = (long long)blockIdx.x * kSlice;
+    float sum = 0.0f;
+    for (int r = q; r < kSlice; r += R) {
+        const long long row = r0 + r;
+        if (row < M) sum += BF ? bf2f(Db[row * N + j]) : Df[row * N + j];
+    }
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    if (q == 0) {
+        float t = 0.0f;
+        for (int k = 0; k < R; k++) t += red[k * N + j];
+        part[(long long)blockIdx.x * N + j] = t;
+    }
 }
 
-// dW1[j, k] = sum_r dZ1[r, j] X[r, k] (FP32)
-__global__ void layer1_wgrad_kernel(const float* __restrict__ dZ, const float* __restrict__ X,
-                                    float* __restrict__ G, int M)
+// layer-1 weight gradient dW1[j][k] = sum_r dZ[r][j] X[r][k] (128 x 16) over a
+// slice: each thread 8 outputs (j = t / 2, k = 8 (t % 2) .. + 8)
+__global__ __launch_bounds__(256) void layer1_wgrad_partial_kernel(const float* __restrict__ dZ,
+                                                                   const float* __restrict__ X,
+                                                                   float* __restrict__ part,
+                                                                   int M)
+{
+    const int j = threadIdx.x >> 1, k0 = 8 * (threadIdx.x & 1);
+    const long long r0 = (long long)blockIdx.x * kSlice;
+    const long long r1 = r0 + kSlice < M ? r0 + kSlice : M;
+    float acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc[k] = 0.0f;
+    for (long long r = r0; r < r1; r++) {
+        const float d = dZ[r * kH + j];
+        const float4 xa = *reinterpret_cast<const float4*>(X + r * kIn + k0);
+        const float4 xb = *reinterpret_cast<const float4*>(X + r * kIn + k0 + 4);
+        acc[0] += d * xa.x; acc[1] += d * xa.y; acc[2] += d * xa.z; acc[3] += d * xa.w;
+        acc[4] += d * xb.x; acc[5] += d * xb.y; acc[6] += d * xb.z; acc[7] += d * xb.w;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) part[(long long)blockIdx.x * (kH * kIn) + j * kIn + k0 + k] = acc[k];
+}
+
+// out[t] = sum over b < nb of part[b][t], t < len (block order)
+__global__ void sum_partials_kernel(const float* __restrict__ part, int nb, int len,
+                                    float* __restrict__ out)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= kH * kIn) return;
-    const int j = t / kIn, k = t % kIn;
+    if (t >= len) return;
     float s = 0.0f;
-    for (int r = 0; r < M; r++) s += dZ[(long long)r * kH + j] * X[(long long)r * kIn + k];
-    G[oW1 + t] = s;
+    for (int b = 0; b < nb; b++) s += part[(long long)b * len + t];
+    out[t] = s;
+}
+
+// Split-K MFMA GEMM for the weight gradients (K = the batch): block z sums
+// k in [z ks, min(K, (z + 1) ks)) of C = A . B into part[z] (M x N, row-major).
+__global__ __launch_bounds__(64) void gemm_bf16_splitk_kernel(
+    const bf16* __restrict__ A, long long sam, long long sak, const bf16* __restrict__ B,
+    long long sbk, long long sbn, int M, int N, int K, int ks, float* __restrict__ part)
+{
+    const int lane = threadIdx.x;
+    const long long i0 = (long long)blockIdx.y * 32, j0 = (long long)blockIdx.x * 32;
+    const long long ia = i0 + (lane & 31), jb = j0 + (lane & 31);
+    const int kq = 4 * (lane >> 5);
+    const int kb = blockIdx.z * ks, ke = kb + ks < K ? kb + ks : K;
+    v16f acc;
+    for (int t = 0; t < 16; t++) acc[t] = 0.0f;
+    for (int k0 = kb; k0 < ke; k0 += 8) {
+        v4s a, b;
+        for (int t = 0; t < 4; t++) {
+            const int k = k0 + kq + t;
+            a[t] = (ia < M && k < ke) ? (short)A[ia * sam + k * sak] : (short)0;
+            b[t] = (jb < N && k < ke) ? (short)B[k * sbk + jb * sbn] : (short)0;
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    const long long jc = j0 + (lane & 31);
+    if (jc >= N) return;
+    float* P = part + (long long)blockIdx.z * M * N;
+    for (int t = 0; t < 16; t++) {
+        const long long ic = i0 + 8 * (t >> 2) + 4 * (lane >> 5) + (t & 3);
+        if (ic < M) P[ic * N + jc] = acc[t];
+    }
 }
 
 // Adam on the FP32 masters (the W4/b4 pads stay zero), then BF16 copies of
@@ -461,6 +534,7 @@ struct disco_surrogate {
     int cap = 0;
     float *X = nullptr, *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr, *Y = nullptr, *T = nullptr;
     float *dA = nullptr, *dZf = nullptr;
+    float* part = nullptr;   // partial sums of the batch reductions
     bf16 *A1 = nullptr, *A2 = nullptr, *A3 = nullptr, *dY = nullptr, *dZ = nullptr;
 };
 
@@ -470,9 +544,9 @@ void free_acts(disco_surrogate* s)
 {
     for (void* p : {(void*)s->X, (void*)s->Z1, (void*)s->Z2, (void*)s->Z3, (void*)s->Y, (void*)s->T,
                     (void*)s->dA, (void*)s->dZf, (void*)s->A1, (void*)s->A2, (void*)s->A3,
-                    (void*)s->dY, (void*)s->dZ})
+                    (void*)s->dY, (void*)s->dZ, (void*)s->part})
         if (p) (void)hipFree(p);
-    s->X = s->Z1 = s->Z2 = s->Z3 = s->Y = s->T = s->dA = s->dZf = nullptr;
+    s->X = s->Z1 = s->Z2 = s->Z3 = s->Y = s->T = s->dA = s->dZf = s->part = nullptr;
     s->A1 = s->A2 = s->A3 = s->dY = s->dZ = nullptr;
     s->cap = 0;
 }
@@ -496,6 +570,8 @@ int ensure_acts(disco_surrogate* s, int rows)
     DISCO_HIP_TRY(ctx, hipMalloc(&s->A3, r * kH * 2));
     DISCO_HIP_TRY(ctx, hipMalloc(&s->dY, r * kOut * 2));
     DISCO_HIP_TRY(ctx, hipMalloc(&s->dZ, r * kH * 2));
+    const size_t slices = (r + kSlice - 1) / kSlice;
+    DISCO_HIP_TRY(ctx, hipMalloc(&s->part, slices * kH * kH * 4));
     s->cap = rows;
     return DISCO_OK;
 }
@@ -508,6 +584,34 @@ void gemm(hipStream_t st, const bf16* A, long long sam, long long sak, const bf1
     const dim3 grid(blocks(N, 32), blocks(M, 32));
     hipLaunchKernelGGL((gemm_bf16_kernel<EPI>), grid, dim3(64), 0, st, A, sam, sak, B, sbk, sbn,
                        M, N, K, bias, C, ldc, Ab, ldab);
+}
+
+// Weight gradient C (M x N, ldc = N) = A . B with K = the batch, split over
+// slices of kSlice rows and summed in slice order.
+void gemm_wgrad(disco_surrogate* s, hipStream_t st, const bf16* A, long long sam, long long sak,
+                const bf16* B, long long sbk, long long sbn, int M, int N, int K, float* C)
+{
+    const int S = (K + kSlice - 1) / kSlice;
+    const dim3 grid(blocks(N, 32), blocks(M, 32), (unsigned)S);
+    hipLaunchKernelGGL(gemm_bf16_splitk_kernel, grid, dim3(64), 0, st, A, sam, sak, B, sbk, sbn,
+                       M, N, K, kSlice, s->part);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(blocks((long long)M * N, 256)), dim3(256), 0, st,
+                       (const float*)s->part, S, M * N, C);
+}
+
+// Column sums of an M x N batch matrix (bf16 or float) into db[N].
+void colsum(disco_surrogate* s, hipStream_t st, const bf16* Db, const float* Df, float* db,
+            int M, int N)
+{
+    const int S = (M + kSlice - 1) / kSlice;
+    if (Db)
+        hipLaunchKernelGGL(colsum_partial_kernel<true>, dim3(S), dim3(256), 0, st, Db, Df,
+                           s->part, M, N);
+    else
+        hipLaunchKernelGGL(colsum_partial_kernel<false>, dim3(S), dim3(256), 0, st, Db, Df,
+                           s->part, M, N);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, (const float*)s->part, S,
+                       N, db);
 }
 
 // Forward of M rows already in s->X; keeps Z1..Z3 when train.
@@ -536,31 +640,32 @@ void backward(disco_surrogate* s, int M, hipStream_t st)
     hipLaunchKernelGGL(loss_kernel, dim3(blocks((long long)M * kOut, 256)), dim3(256), 0, st,
                        s->Y, s->T, s->dY, s->loss, M);
     // layer 4: dW4 = dY^T A3 (32 x 128), db4, dA3 = dY W4 (M x 128)
-    gemm<0>(st, s->dY, 1, kOut, s->A3, kH, 1, kOut, kH, M, nullptr, s->G + oW4, kH, nullptr, 0);
-    hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(64), 0, st, s->dY, (const float*)nullptr,
-                       s->G + oB4, M, kOut);
+    gemm_wgrad(s, st, s->dY, 1, kOut, s->A3, kH, 1, kOut, kH, M, s->G + oW4);
+    colsum(s, st, s->dY, nullptr, s->G + oB4, M, kOut);
     gemm<0>(st, s->dY, kOut, 1, W4, kH, 1, M, kH, kOut, nullptr, s->dA, kH, nullptr, 0);
     // layer 3
     hipLaunchKernelGGL(silu_bwd_kernel, dim3(blocks((long long)M * kH, 256)), dim3(256), 0, st,
                        s->dA, s->Z3, s->dZ, (float*)nullptr, (long long)M * kH);
-    gemm<0>(st, s->dZ, 1, kH, s->A2, kH, 1, kH, kH, M, nullptr, s->G + oW3, kH, nullptr, 0);
-    hipLaunchKernelGGL(colsum_kernel, dim3(2), dim3(64), 0, st, s->dZ, (const float*)nullptr,
-                       s->G + oB3, M, kH);
+    gemm_wgrad(s, st, s->dZ, 1, kH, s->A2, kH, 1, kH, kH, M, s->G + oW3);
+    colsum(s, st, s->dZ, nullptr, s->G + oB3, M, kH);
     gemm<0>(st, s->dZ, kH, 1, W3, kH, 1, M, kH, kH, nullptr, s->dA, kH, nullptr, 0);
     // layer 2
     hipLaunchKernelGGL(silu_bwd_kernel, dim3(blocks((long long)M * kH, 256)), dim3(256), 0, st,
                        s->dA, s->Z2, s->dZ, (float*)nullptr, (long long)M * kH);
-    gemm<0>(st, s->dZ, 1, kH, s->A1, kH, 1, kH, kH, M, nullptr, s->G + oW2, kH, nullptr, 0);
-    hipLaunchKernelGGL(colsum_kernel, dim3(2), dim3(64), 0, st, s->dZ, (const float*)nullptr,
-                       s->G + oB2, M, kH);
+    gemm_wgrad(s, st, s->dZ, 1, kH, s->A1, kH, 1, kH, kH, M, s->G + oW2);
+    colsum(s, st, s->dZ, nullptr, s->G + oB2, M, kH);
     gemm<0>(st, s->dZ, kH, 1, W2, kH, 1, M, kH, kH, nullptr, s->dA, kH, nullptr, 0);
     // layer 1 (FP32)
     hipLaunchKernelGGL(silu_bwd_kernel, dim3(blocks((long long)M * kH, 256)), dim3(256), 0, st,
                        s->dA, s->Z1, (bf16*)nullptr, s->dZf, (long long)M * kH);
-    hipLaunchKernelGGL(layer1_wgrad_kernel, dim3(blocks(kH * kIn, 256)), dim3(256), 0, st,
-                       s->dZf, s->X, s->G, M);
-    hipLaunchKernelGGL(colsum_kernel, dim3(2), dim3(64), 0, st, (const bf16*)nullptr, s->dZf,
-                       s->G + oB1, M, kH);
+    {
+        const int S = (M + kSlice - 1) / kSlice;
+        hipLaunchKernelGGL(layer1_wgrad_partial_kernel, dim3(S), dim3(256), 0, st,
+                           (const float*)s->dZf, (const float*)s->X, s->part, M);
+        hipLaunchKernelGGL(sum_partials_kernel, dim3(blocks(kH * kIn, 256)), dim3(256), 0, st,
+                           (const float*)s->part, S, kH * kIn, s->G + oW1);
+    }
+    colsum(s, st, nullptr, s->dZf, s->G + oB1, M, kH);
 }
 
 int upload_rows(disco_surrogate* s, const float* feats, const float* targets, int n)
