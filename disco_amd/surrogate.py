@@ -321,7 +321,7 @@ def grid_nodes(x, y, z, h):
 
 
 def base_far_shell_torch(ctx, x, y, z, loops, box, img, kind="rule7", u=None,
-                         accumulate=False, prec=B.PREC_FP32C, grid_h=None):
+                         accumulate=False, prec=B.PREC_FP32C, grid_h=None, grid_groups=8):
     """Far shell of the loops' loopon rule (`kind`), summed by the loopon kernels:
     at every atom, or (grid_h set) at the nodes of a grid of that spacing over
     the atoms and interpolated to them -- the far shell is smooth on the scale of
@@ -329,18 +329,38 @@ def base_far_shell_torch(ctx, x, y, z, loops, box, img, kind="rule7", u=None,
     import torch
     if u is None:
         u = [torch.zeros_like(x) for _ in range(3)]
-    plan = B.Plan(ctx, base_loopons(loops, kind), box, img, prec, kind="loopon")
-    set_inner(plan, 1)
+    base = base_loopons(loops, kind)
     if grid_h is None:
+        plan = B.Plan(ctx, base, box, img, prec, kind="loopon")
+        set_inner(plan, 1)
         plan.run_torch(x, y, z, *u, accumulate=accumulate)
-    else:
-        origin, dims, nodes = grid_nodes(x, y, z, grid_h)
-        gu = [torch.zeros_like(nodes[0]) for _ in range(3)]
-        plan.run_torch(*nodes, *gu)
-        shape = (dims[2], dims[1], dims[0])
-        B.grid_interp_torch(ctx, x, y, z, *u, [t.view(shape) for t in gu], origin, grid_h,
-                            accumulate=accumulate)
-    plan.free()
+        plan.free()
+        return u
+    origin, dims, nodes = grid_nodes(x, y, z, grid_h)
+    # a grid holds few nodes (tens of thousands) for the GPU's 256 CUs: the loops
+    # are split into groups summed concurrently on their own streams, then added
+    # in group order
+    groups = max(1, min(grid_groups, len(base)))
+    main = torch.cuda.current_stream()
+    plans = []
+    for g in range(groups):                 # every upload before the first launch
+        plan = B.Plan(ctx, base[g::groups], box, img, prec, kind="loopon")
+        set_inner(plan, 1)
+        plans.append(plan)
+    parts = [[torch.zeros_like(nodes[0]) for _ in range(3)] for _ in plans]
+    sides = [torch.cuda.Stream(device=x.device) for _ in plans]
+    for plan, out, side in zip(plans, parts, sides):
+        side.wait_stream(main)
+        plan.run_torch(*nodes, *out, stream=side.cuda_stream)
+    for side in sides:
+        main.wait_stream(side)
+    gu = [sum(p[k] for p in parts) for k in range(3)]
+    shape = (dims[2], dims[1], dims[0])
+    B.grid_interp_torch(ctx, x, y, z, *u, [t.view(shape) for t in gu], origin, grid_h,
+                        accumulate=accumulate)
+    torch.cuda.synchronize()            # the plans' tables are freed below
+    for plan in plans:
+        plan.free()
     return u
 
 
