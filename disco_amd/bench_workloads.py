@@ -100,15 +100,30 @@ def run_c4(args):
     import torch
     from . import backend as B
     from . import geometry as G
-    t0 = time.time()
-    samples = G.config_c4()
-    print(f"[c4] {len(samples)} samples, {sum(s.n for s in samples)} atoms "
-          f"({time.time() - t0:.1f} s)", flush=True)
-    dev = torch.device("cuda:0")
-    cat = lambda k: torch.from_numpy(np.concatenate([getattr(s, k) for s in samples])).to(dev)
-    x, y, z = cat("x"), cat("y"), cat("z")
-    u = [torch.zeros_like(x) for _ in range(3)]
     ctx = B.Context(0)
+    G._nlib()
+    # sample generation: loop contours by the C++ producer, lattices on the device
+    t0 = time.perf_counter()
+    samples = G.config_c4_specs()
+    t_host = time.perf_counter() - t0
+    dev = torch.device("cuda:0")
+    ntot = sum(s.n for s in samples)
+    x, y, z = (torch.empty(ntot, dtype=torch.float64, device=dev) for _ in range(3))
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    off = 0
+    for s in samples:
+        N = s.meta["cells"]
+        B.lattice_torch(ctx, "BCC", N, N, N, s.meta["a0"],
+                        out=[t[off:off + s.n] for t in (x, y, z)])
+        off += s.n
+    e1.record()
+    torch.cuda.synchronize()
+    t_lat = e0.elapsed_time(e1)
+    print(f"[c4] {len(samples)} samples, {ntot} atoms: contours {t_host:.3f} s on the host "
+          f"(C++ producer), lattices {t_lat:.2f} ms on the device", flush=True)
+    u = [torch.zeros_like(x) for _ in range(3)]
     bt = B.Batch(ctx, samples, 1, B.PREC_FP32C)
     evals = bt.evals()
     default_shape = bt.tuning
@@ -136,15 +151,19 @@ def run_c4(args):
     ms = e0.elapsed_time(e1) / args.steps
     # accuracy: FP64 sums of every 16th sample
     errs = []
+    host = lambda k: [t[bt.offsets[k]:bt.offsets[k] + samples[k].n].cpu().numpy()
+                      for t in (x, y, z)]
     for k in range(0, len(samples), 16):
         s = samples[k]
         off = bt.offsets[k]
-        ref = B.line_sum(ctx, s.x, s.y, s.z, s.loops, s.box, 1, B.PREC_FP64)
+        ref = B.line_sum(ctx, *host(k), s.loops, s.box, 1, B.PREC_FP64)
         got = [t[off:off + s.n].cpu().numpy() for t in u]
         errs.append(max(float(np.abs(a - b).max()) for a, b in zip(got, ref)))
     cpu = None
     if not args.no_cpu:
-        v, c, desc = _cpu_line(samples[0], 64)
+        hx, hy, hz = host(0)
+        s0 = G.Sample("C4", hx, hy, hz, samples[0].loops, samples[0].box, 1)
+        v, c, desc = _cpu_line(s0, 64)
         cpu = {"value": v, "unit": "evals/s", "cores": c, "kind": "port",
                "sample": "sample 0 of the batch, " + desc}
     ach = evals * FLOP_LINE / (ms * 1e-3) / 1e12
@@ -155,7 +174,8 @@ def run_c4(args):
            "data": "synthetic",
            "config": {"workload": "256 BCC samples (20-60 cells, 1-16 loops, 16-128 segments), +-1",
                       "atoms": int(sum(s.n for s in samples)), "evals_per_step": evals,
-                      "tuning": best, "default_shape_ms": ms_default, "autotune_s": tune_s},
+                      "tuning": best, "default_shape_ms": ms_default, "autotune_s": tune_s,
+                      "datagen": {"contours_host_s": t_host, "lattices_device_ms": t_lat}},
            "roofline": {"bound": "valu", "achieved": ach, "peak": FP32_PEAK, "unit": "TFLOP/s",
                         "frac": ach / FP32_PEAK, "traffic": None,
                         "kernel": "batch_line_kernel (far + near passes)", "kernel_ms": ms},

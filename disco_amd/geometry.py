@@ -408,10 +408,9 @@ def config_c3(seed: int = 20243, nloops: int = 20, da: float = 1.585):
                   {"lattice": "BCC", "a0": a0, "cells": N, "seed": seed, "da": da})
 
 
-def config_c4_sample(rng, a0: float = 2.8553):
-    """One sample of the batched workload: BCC side U{20..60} cells, 1-16 loops
-    of 16-128 segments (circles, rectangles, parallelograms), size U(8, 40)
-    capped at a third of the box, images +-1."""
+def _c4_draw(rng, a0: float):
+    """Random draws of one batched-workload sample: the side N (cells) and its
+    loops as (Loop, nseg for a circle, subdivide for a sharp shape)."""
     N = int(rng.integers(20, 61))
     L = N * a0
     nl = int(rng.integers(1, 17))
@@ -425,11 +424,11 @@ def config_c4_sample(rng, a0: float = 2.8553):
         c, _ = snap_to_mid_plane(c0, n, a0, N, N, N, "BCC")
         kind = int(rng.integers(3))
         if kind == 0:
-            loops.append(Loop("circle", c, list(m), list(n), a0, radius=size).line_data(nseg=nseg))
+            loops.append((Loop("circle", c, list(m), list(n), a0, radius=size), nseg, None))
         elif kind == 1:
             lp = Loop("rect", c, list(m), list(n), a0, half_m=size,
                       half_p=size * float(rng.uniform(0.5, 1.0)))
-            loops.append(lp.line_data(subdivide=max(nseg, 4)))
+            loops.append((lp, None, max(nseg, 4)))
         else:
             u, _v = default_axes(_unit(m), _unit(n))
             w = _cross(_unit(n), u)
@@ -437,11 +436,185 @@ def config_c4_sample(rng, a0: float = 2.8553):
             d2 = [math.cos(ang) * u[i] + math.sin(ang) * w[i] for i in range(3)]
             lp = Loop("parallelogram", c, list(m), list(n), a0, half_m=size,
                       half_p=size * float(rng.uniform(0.5, 1.0)), dir1=u, dir2=d2)
-            loops.append(lp.line_data(subdivide=max(nseg, 4)))
+            loops.append((lp, None, max(nseg, 4)))
+    return N, loops
+
+
+def config_c4_sample(rng, a0: float = 2.8553):
+    """One sample of the batched workload: BCC side U{20..60} cells, 1-16 loops
+    of 16-128 segments (circles, rectangles, parallelograms), size U(8, 40)
+    capped at a third of the box, images +-1."""
+    N, draws = _c4_draw(rng, a0)
+    loops = [lp.line_data(nseg=ns) if ns else lp.line_data(subdivide=sd) for lp, ns, sd in draws]
     x, y, z = lattice("BCC", N, N, N, a0)
+    L = N * a0
     return Sample("C4", x, y, z, loops, (L, L, L), 1, {"lattice": "BCC", "a0": a0, "cells": N})
+
+
+@dataclass
+class SampleSpec:
+    """A sample whose atoms are generated on the device: its lattice, loops, box."""
+    name: str
+    n: int
+    loops: list
+    box: tuple
+    img: int
+    meta: dict = field(default_factory=dict)
+
+
+def config_c4_specs(seed: int = 20244, nsamples: int = 256, a0: float = 2.8553):
+    """The batched workload (same draws as config_c4) with the loop contours built
+    by the C++ producer in one call and no host atoms: each sample's lattice is
+    generated on the device (backend.lattice_torch)."""
+    rng = np.random.default_rng(seed)
+    draws = [_c4_draw(rng, a0) for _ in range(nsamples)]
+    flat = [d for _, ds in draws for d in ds]
+    V, off = native_polylines([lp for lp, _, _ in flat], 0.01, [ns for _, ns, _ in flat],
+                              [sd for _, _, sd in flat])
+    out, k = [], 0
+    for N, ds in draws:
+        loops = []
+        for lp, _, _ in ds:
+            P = V[off[k]:off[k + 1]]
+            loops.append(LoopData(phys=lp.phys(), line=True, vx=P[:, 0].copy(),
+                                  vy=P[:, 1].copy(), vz=P[:, 2].copy(), cx=lp.center[0],
+                                  cy=lp.center[1], cz=lp.center[2]))
+            k += 1
+        L = N * a0
+        out.append(SampleSpec("C4", 2 * N * N * N, loops, (L, L, L), 1,
+                              {"lattice": "BCC", "a0": a0, "cells": N}))
+    return out
 
 
 def config_c4(seed: int = 20244, nsamples: int = 256):
     rng = np.random.default_rng(seed)
     return [config_c4_sample(rng) for _ in range(nsamples)]
+
+
+# ----------------------------------------------------------------- native producer
+# The same producer in C++ (include/disco_geometry.h), for batched sample
+# generation: thousands of loops per batch without a Python loop per vertex.
+import ctypes as _ct
+
+
+class _Shape(_ct.Structure):
+    _fields_ = [("shape", _ct.c_int), ("center", _ct.c_double * 3), ("m", _ct.c_double * 3),
+                ("n", _ct.c_double * 3), ("a0", _ct.c_double), ("radius", _ct.c_double),
+                ("half_m", _ct.c_double), ("half_p", _ct.c_double),
+                ("dir1", _ct.c_double * 3), ("dir2", _ct.c_double * 3), ("nvert", _ct.c_int),
+                ("vert", _ct.POINTER(_ct.c_double))]
+
+
+_SHAPES = {"circle": 0, "polygon": 1, "rect": 2, "parallelogram": 3}
+_native = None
+
+
+def _nlib():
+    global _native
+    if _native is None:
+        from .backend import lib
+        L = lib()
+        dp, vp, i, d = _ct.POINTER(_ct.c_double), _ct.c_void_p, _ct.c_int, _ct.c_double
+        L.disco_geo_regular_polygon.argtypes = [i, d, dp, dp, dp]
+        L.disco_geo_snap_mid_plane.argtypes = [dp, dp, d, i, i, i, i, dp, dp]
+        L.disco_geo_polyline.argtypes = [_ct.POINTER(_Shape), d, i, i, dp, dp, dp, i]
+        L.disco_geo_loopons.argtypes = [_ct.POINTER(_Shape), d, dp, dp, dp, i]
+        L.disco_geo_phys.argtypes = [_ct.POINTER(_Shape), d, dp]
+        L.disco_geo_polylines.argtypes = [_ct.POINTER(_Shape), i, d, _ct.POINTER(i),
+                                          _ct.POINTER(i), _ct.POINTER(_ct.c_longlong), dp, dp,
+                                          dp, _ct.c_longlong]
+        L.disco_geo_polylines.restype = _ct.c_longlong
+        _native = L
+    return _native
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_ct.POINTER(_ct.c_double))
+
+
+def native_shape(lp: Loop):
+    """The C struct of a loop (keeps the polygon vertices alive with it)."""
+    s = _Shape()
+    s.shape = _SHAPES[lp.shape]
+    s.center[:] = [float(t) for t in lp.center]
+    s.m[:] = [float(t) for t in lp.m]
+    s.n[:] = [float(t) for t in lp.n]
+    s.a0, s.radius, s.half_m, s.half_p = lp.a0, lp.radius, lp.half_m, lp.half_p
+    if lp.shape == "parallelogram":
+        s.dir1[:] = [float(t) for t in lp.dir1]
+        s.dir2[:] = [float(t) for t in lp.dir2]
+    keep = None
+    if lp.shape == "polygon":
+        keep = np.ascontiguousarray(np.asarray(lp.vertices, dtype=np.float64).reshape(-1))
+        s.nvert = len(lp.vertices)
+        s.vert = _dptr(keep)
+    return s, keep
+
+
+def _call_points(fn, *args):
+    n = fn(*args, None, None, None, 0)
+    if n < 0:
+        raise ValueError("invalid loop shape")
+    x, y, z = np.zeros(n), np.zeros(n), np.zeros(n)
+    fn(*args, _dptr(x), _dptr(y), _dptr(z), n)
+    return np.stack([x, y, z], 1)
+
+
+def native_polyline(lp: Loop, eps: float = 0.01, nseg: int | None = None,
+                    subdivide: int | None = None):
+    s, keep = native_shape(lp)
+    return _call_points(_nlib().disco_geo_polyline, _ct.byref(s), float(eps), int(nseg or 0),
+                        int(subdivide or 0))
+
+
+def native_loopons(lp: Loop, da: float):
+    s, keep = native_shape(lp)
+    return _call_points(_nlib().disco_geo_loopons, _ct.byref(s), float(da))
+
+
+def native_phys(lp: Loop, da: float = 0.0):
+    s, keep = native_shape(lp)
+    out = np.zeros(10)
+    if _nlib().disco_geo_phys(_ct.byref(s), float(da), _dptr(out)) < 0:
+        raise ValueError("invalid loop shape")
+    return tuple(out)
+
+
+def native_regular_polygon(sides: int, R: float, m, n):
+    out = np.zeros(3 * sides)
+    mm, nn = np.asarray(m, dtype=np.float64), np.asarray(n, dtype=np.float64)
+    if _nlib().disco_geo_regular_polygon(sides, float(R), _dptr(mm), _dptr(nn), _dptr(out)) < 0:
+        raise ValueError("a polygon loop needs Sides >= 3")
+    return out.reshape(sides, 3)
+
+
+def native_snap_to_mid_plane(center, n, a0, nx, ny, nz, kind):
+    c, nn = np.asarray(center, dtype=np.float64), np.asarray(n, dtype=np.float64)
+    out, sp = np.zeros(3), np.zeros(1)
+    if _nlib().disco_geo_snap_mid_plane(_dptr(c), _dptr(nn), float(a0), nx, ny, nz,
+                                        1 if kind == "FCC" else 0, _dptr(out), _dptr(sp)) < 0:
+        raise ValueError("loop normal n must be non-zero integer Miller indices")
+    return list(out), float(sp[0])
+
+
+def native_polylines(loops, eps: float = 0.01, nseg=None, subdivide=None):
+    """Many loops' polylines in one call: (vertices (N x 3), offsets (nloops + 1))."""
+    shapes = (_Shape * max(1, len(loops)))()
+    keep = []
+    for k, lp in enumerate(loops):
+        s, kp = native_shape(lp)
+        shapes[k] = s
+        keep.append(kp)
+    nl = len(loops)
+    ns = (_ct.c_int * max(1, nl))(*[int(v or 0) for v in (nseg or [0] * nl)])
+    sd = (_ct.c_int * max(1, nl))(*[int(v or 0) for v in (subdivide or [0] * nl)])
+    off = np.zeros(nl + 1, dtype=np.int64)
+    offp = off.ctypes.data_as(_ct.POINTER(_ct.c_longlong))
+    L = _nlib()
+    total = L.disco_geo_polylines(shapes, nl, float(eps), ns, sd, offp, None, None, None, 0)
+    if total < 0:
+        raise ValueError("invalid loop shape")
+    x, y, z = np.zeros(total), np.zeros(total), np.zeros(total)
+    L.disco_geo_polylines(shapes, nl, float(eps), ns, sd, offp, _dptr(x), _dptr(y), _dptr(z),
+                          total)
+    return np.stack([x, y, z], 1), off

@@ -1,0 +1,84 @@
+/* disco_geometry.h -- host producer of the backend's loop inputs (C ABI).
+ *
+ * The loop geometry that DISCO's driver builds before every sum -- closed
+ * polylines, loopon grids, the mid-plane snap of loop centres -- restated in
+ * C++ for batched sample generation (many samples, thousands of loops): the
+ * same formulas and operation order as the reference's loop code, so the
+ * vertices equal the reference's (tested against the compiled reference's
+ * outputs).  Host code only; no device is needed.
+ *
+ * Reference interfaces replaced:
+ *   disco_geo_regular_polygon  regularPolygon      (Code/loop.cpp:585-608)
+ *   disco_geo_snap_mid_plane   snapToMidPlane      (Code/loop.cpp:342-374)
+ *   disco_geo_polyline         loop::polyline      (Code/loop.cpp:536-580), sharp
+ *                              shapes and circles
+ *   disco_geo_loopons          loop::generateLoopons / insideUV
+ *                              (Code/loop.cpp:253-316), sharp shapes
+ *   disco_geo_phys             the LoopPhys of a loop (Code/loop.cpp:25-29)
+ *   disco_geo_polylines        the driver's per-loop LoopData assembly
+ *                              (Code/main.cpp:689-711), for many loops at once
+ */
+#ifndef DISCO_GEOMETRY_H
+#define DISCO_GEOMETRY_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DISCO_SHAPE_CIRCLE 0
+#define DISCO_SHAPE_POLYGON 1        /* vertices given as offsets from the centre */
+#define DISCO_SHAPE_RECT 2
+#define DISCO_SHAPE_PARALLELOGRAM 3
+
+#define DISCO_LATTICE_BCC_GEO 0
+#define DISCO_LATTICE_FCC_GEO 1
+
+typedef struct {
+    int shape;                 /* DISCO_SHAPE_*                                        */
+    double center[3];
+    double m[3];               /* Burgers direction in units of a0/2: |b| = a0 |m| / 2 */
+    double n[3];               /* plane normal                                         */
+    double a0;
+    double radius;             /* circle                                               */
+    double half_m, half_p;     /* rect, parallelogram: half extents along the axes     */
+    double dir1[3], dir2[3];   /* parallelogram edge directions                        */
+    int nvert;                 /* polygon                                              */
+    const double *vert;        /* polygon: 3 * nvert offsets from the centre           */
+} disco_loop_shape;
+
+/* Return values: >= 0 the number of points (written when <= cap), < 0 an
+ * error: DISCO_GEO_ERR_ARG for an invalid shape or argument. */
+#define DISCO_GEO_ERR_ARG (-1)
+
+/* Vertices (3 x sides doubles, xyz interleaved) of a regular polygon of
+ * circumradius R in the plane of n, as offsets from its centre. */
+int disco_geo_regular_polygon(int sides, double R, const double m[3], const double n[3],
+                              double *out);
+/* Centre moved along n onto the nearest mid-plane between (hkl) planes of the
+ * box-centred lattice (nx, ny, nz cells); n integer Miller indices. */
+int disco_geo_snap_mid_plane(const double c[3], const double n[3], double a0, int nx, int ny,
+                             int nz, int lattice, double out[3], double *spacing);
+/* Closed contour, counter-clockwise about n: a circle as nseg chords of the
+ * area-preserving radius (nseg <= 0: as many as the sagitta eps needs), a
+ * sharp shape by its corners, or (subdivide > 0) each side cut into collinear
+ * pieces so the contour has `subdivide` segments. */
+int disco_geo_polyline(const disco_loop_shape *s, double eps, int nseg, int subdivide,
+                       double *vx, double *vy, double *vz, int cap);
+/* Loopon grid of spacing da inside the loop. */
+int disco_geo_loopons(const disco_loop_shape *s, double da, double *x, double *y, double *z,
+                      int cap);
+/* phys = (|b|, |b|, |b|, m/|m|, n/|n|, dS) with dS = da^2 (times |dir1 x dir2| for a
+ * parallelogram), or 0 for da <= 0. */
+int disco_geo_phys(const disco_loop_shape *s, double da, double phys[10]);
+/* Many loops' polylines into concatenated arrays in one call:
+ * offsets[l] .. offsets[l + 1] are loop l's vertices; nseg / subdivide per
+ * loop (may be NULL: 0).  Returns the total vertex count (arrays written when
+ * it is <= cap; offsets always). */
+long long disco_geo_polylines(const disco_loop_shape *shapes, int nloops, double eps,
+                              const int *nseg, const int *subdivide, long long *offsets,
+                              double *vx, double *vy, double *vz, long long cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DISCO_GEOMETRY_H */

@@ -1,6 +1,6 @@
-"""The C-ABI library: it loads, exports every entry point include/disco_hip.h
-declares, its struct layout matches the Python bindings, and argument errors
-are reported without a device (no compute call is made here)."""
+"""The C-ABI library: it loads, exports every entry point include/*.h declares,
+its struct layouts match the Python bindings, and argument errors are reported
+without a device (no compute call is made here)."""
 import ctypes
 import os
 import re
@@ -12,19 +12,24 @@ import pytest
 from disco_amd import backend as B
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "disco_hip.h")
+INCLUDE = os.path.join(ROOT, "include")
 
 
 def _declared():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|double|long long|const char \*)\s*(disco_hip_\w+)\s*\(",
-                                 text, flags=re.M)))
+    names = set()
+    for h in sorted(os.listdir(INCLUDE)):
+        if h.endswith(".h"):
+            text = open(os.path.join(INCLUDE, h)).read()
+            names |= set(re.findall(r"^\s*(?:int|double|long long|const char \*)\s*"
+                                    r"(disco_(?:hip|geo)_\w+)\s*\(", text, flags=re.M))
+    return sorted(names)
 
 
 def test_header_declares_the_entry_points():
     names = _declared()
     for must in ("disco_hip_line_sum", "disco_hip_loopon_sum", "disco_hip_plan_line",
-                 "disco_hip_plan_run", "disco_hip_device_count", "disco_hip_open"):
+                 "disco_hip_plan_run", "disco_hip_device_count", "disco_hip_open",
+                 "disco_geo_polyline", "disco_geo_polylines", "disco_geo_snap_mid_plane"):
         assert must in names
 
 
@@ -46,12 +51,30 @@ def test_struct_layout_matches_the_bindings(tmp_path):
                    'sizeof(disco_timing), sizeof(disco_device_info),'
                    'offsetof(disco_device_info, arch));return 0;}\n')
     exe = tmp_path / "layout"
-    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    subprocess.run([cc, "-I", INCLUDE, str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
     want = [ctypes.sizeof(B._LineLoop), B._LineLoop.c.offset, ctypes.sizeof(B._LooponLoop),
             B._LooponLoop.phys.offset, ctypes.sizeof(B._Timing), ctypes.sizeof(B._DeviceInfo),
             B._DeviceInfo.arch.offset]
     assert got == want
+
+
+def test_shape_layout_matches_the_geometry_binding(tmp_path):
+    from disco_amd import geometry as G
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if not cc:
+        pytest.skip("no C compiler")
+    src = tmp_path / "shape.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "disco_geometry.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu\\n",'
+                   'sizeof(disco_loop_shape), offsetof(disco_loop_shape, a0),'
+                   'offsetof(disco_loop_shape, dir2), offsetof(disco_loop_shape, nvert),'
+                   'offsetof(disco_loop_shape, vert));return 0;}\n')
+    exe = tmp_path / "shape"
+    subprocess.run([cc, "-I", INCLUDE, str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    S = G._Shape
+    assert got == [ctypes.sizeof(S), S.a0.offset, S.dir2.offset, S.nvert.offset, S.vert.offset]
 
 
 def test_error_codes_without_a_device():
