@@ -30,7 +30,8 @@ PREC_FP64 = 0
 PREC_FP32C = 1
 
 ERR_NAMES = {0: "DISCO_OK", 1: "DISCO_ERR_ARG", 2: "DISCO_ERR_NO_DEVICE", 3: "DISCO_ERR_HIP",
-             4: "DISCO_ERR_OOM", 5: "DISCO_ERR_TOO_LARGE", 6: "DISCO_ERR_STATE"}
+             4: "DISCO_ERR_OOM", 5: "DISCO_ERR_TOO_LARGE", 6: "DISCO_ERR_STATE",
+             7: "DISCO_ERR_IO"}
 
 
 # ------------------------------------------------------------------ types

@@ -16,6 +16,13 @@ struct disco_ctx {
     // cached atom buffers (grow-only)
     double* d_atoms = nullptr;   // x | y | z | ux | uy | uz
     size_t cap_atoms = 0;
+    // output stage (grow-only): per-block byte counts and offsets, a device
+    // text buffer, and two pinned staging buffers for file writes
+    void* d_text_scratch = nullptr;
+    size_t cap_text_scratch = 0;
+    char* d_text = nullptr;
+    size_t cap_text = 0;
+    char* h_stage[2] = {nullptr, nullptr};
 };
 
 namespace disco {
@@ -25,6 +32,9 @@ inline int fail(disco_ctx* ctx, int code, const std::string& msg)
     if (ctx) ctx->err = msg;
     return code;
 }
+
+// Frees the output stage's buffers (writers.hip); called by disco_hip_close.
+void release_output_buffers(disco_ctx* ctx);
 
 inline double now_s()
 {

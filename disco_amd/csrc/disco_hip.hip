@@ -956,6 +956,7 @@ const char* disco_hip_strerror(int code)
         case DISCO_ERR_OOM: return "out of device memory";
         case DISCO_ERR_TOO_LARGE: return "size beyond a documented maximum";
         case DISCO_ERR_STATE: return "call out of order";
+        case DISCO_ERR_IO: return "file could not be written";
         default: return "unknown error code";
     }
 }
@@ -1074,6 +1075,7 @@ int disco_hip_close(disco_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->d_atoms) (void)hipFree(ctx->d_atoms);
+    disco::release_output_buffers(ctx);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

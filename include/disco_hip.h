@@ -32,7 +32,8 @@ enum {
     DISCO_ERR_HIP = 3,        /* a HIP runtime call failed (detail: last_error)      */
     DISCO_ERR_OOM = 4,        /* device allocation failed                            */
     DISCO_ERR_TOO_LARGE = 5,  /* a size beyond a documented maximum                  */
-    DISCO_ERR_STATE = 6       /* call out of order (e.g. surrogate not trained)      */
+    DISCO_ERR_STATE = 6,      /* call out of order (e.g. surrogate not trained)      */
+    DISCO_ERR_IO = 7          /* a file could not be opened or written               */
 };
 
 /* Arithmetic of the pair terms. */

@@ -30,6 +30,15 @@ loopon_sum (+-img images)     golden kroupaCPU output at +-1 (C3 loop, 4,096
                               atoms); +-N = sum of single-image fields
 line_sum_ref32                golden barnettLoop<float> vectors (the float
                               instantiation the reference's --fp32 uses)
+final_positions               exact rational arithmetic of the reference's
+                              expression shapes under GCC contraction; wrap
+                              closed forms; identity U leaves wrapped + L/2
+lammps_box                    exact rational Gram-Schmidt; orthogonal boxes
+                              give (dx, dy, dz) and the unit frame
+fixed8, write_xyz,            Python's correctly rounded '%.8f' (an
+write_lammps                  independent implementation) on random values,
+                              exact ties, -0.0, tiny negatives, nan/inf; the
+                              file layout line by line
 ============================  =================================================
 """
 from __future__ import annotations
@@ -41,6 +50,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(_HERE, "liboracle.so")
+OUT_SO = os.path.join(_HERE, "liboracle_out.so")
 REF_SO = os.path.join(_HERE, "_ref", "libdisco_ref.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -52,6 +62,7 @@ _f = ctypes.c_float
 
 _lib = None
 _ref = None
+_out = None
 
 
 def _ptr(a, kind=_dp):
@@ -298,3 +309,80 @@ def ref_lattice(lattice, nx, ny, nz, a0):
     x, y, z = np.zeros(cap), np.zeros(cap), np.zeros(cap)
     ref().ref_lattice(1 if lattice == "FCC" else 0, nx, ny, nz, a0, _ptr(x), _ptr(y), _ptr(z), cap)
     return x, y, z
+
+
+# ---------------------------------------------------------------- output stage
+def out_lib():
+    """The output-stage restatement (raises if it was not built)."""
+    global _out
+    if _out is None:
+        if not os.path.exists(OUT_SO):
+            raise RuntimeError(f"oracle library missing: {OUT_SO} (run `make -C oracle`)")
+        L = ctypes.CDLL(OUT_SO)
+        _ll = ctypes.c_longlong
+        L.oracle_final_positions.argtypes = [_ll, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                             _dp, _dp]
+        L.oracle_lammps_box.argtypes = [_dp, _dp, _dp]
+        L.oracle_write_xyz.argtypes = [ctypes.c_char_p, _ll, _dp, _dp, _dp, _dp,
+                                       ctypes.c_char_p, _ip, _ip]
+        L.oracle_write_xyz.restype = _i
+        L.oracle_write_lammps.argtypes = [ctypes.c_char_p, _ll, _dp, _dp, _dp, _dp,
+                                          ctypes.c_char_p, _d]
+        L.oracle_write_lammps.restype = _i
+        L.oracle_fixed8.argtypes = [_d, ctypes.c_char_p, _i]
+        L.oracle_fixed8.restype = _i
+        _out = L
+    return _out
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def final_positions(x, y, z, ux, uy, uz, box, U=None):
+    """Code/main.cpp:797-818: x + u wrapped into the box centred at the origin,
+    shifted by +L/2, times U (row-major; identity by default)."""
+    x, y, z, ux, uy, uz = (_c(a) for a in (x, y, z, ux, uy, uz))
+    U = _c(np.eye(3) if U is None else U).reshape(9)
+    box = _c(box)
+    f = [np.empty_like(x) for _ in range(3)]
+    out_lib().oracle_final_positions(x.size, *(_ptr(a) for a in (x, y, z, ux, uy, uz)),
+                                     _ptr(box), _ptr(U), *(_ptr(a) for a in f))
+    return f
+
+
+def lammps_box(Hp):
+    """Code/main.cpp:64-86: (lx, ly, lz, xy, xz, yz) and the frame rows e1, e2, e3."""
+    Hp = _c(Hp).reshape(9)
+    b, fr = np.empty(6), np.empty(9)
+    out_lib().oracle_lammps_box(_ptr(Hp), _ptr(b), _ptr(fr))
+    return b, fr.reshape(3, 3)
+
+
+def write_xyz(path, fx, fy, fz, Hp, element, grain=None, twin=None):
+    fx, fy, fz = _c(fx), _c(fy), _c(fz)
+    g = None if grain is None else _c(grain, np.int32)
+    t = None if twin is None else _c(twin, np.int32)
+    r = out_lib().oracle_write_xyz(str(path).encode(), fx.size, _ptr(fx), _ptr(fy), _ptr(fz),
+                                   _ptr(_c(Hp).reshape(9)), element.encode(),
+                                   None if g is None else _ptr(g, _ip),
+                                   None if t is None else _ptr(t, _ip))
+    if r:
+        raise OSError(f"oracle could not write {path}")
+
+
+def write_lammps(path, fx, fy, fz, Hp, element, mass):
+    fx, fy, fz = _c(fx), _c(fy), _c(fz)
+    r = out_lib().oracle_write_lammps(str(path).encode(), fx.size, _ptr(fx), _ptr(fy), _ptr(fz),
+                                      _ptr(_c(Hp).reshape(9)), element.encode(), float(mass))
+    if r:
+        raise OSError(f"oracle could not write {path}")
+
+
+def fixed8(v) -> str:
+    """One value as the writers print it (iostream fixed, 8 decimals)."""
+    buf = ctypes.create_string_buffer(400)
+    n = out_lib().oracle_fixed8(float(v), buf, 400)
+    if n < 0:
+        raise ValueError("value too long")
+    return buf.value.decode()
