@@ -15,6 +15,7 @@ Each prints one JSON line in bench.py's format.  The default bench line (the
 from __future__ import annotations
 
 import json
+import os
 import time
 
 import numpy as np
@@ -351,9 +352,132 @@ def run_lattice(args):
     ctx.close()
 
 
+def run_writers(args):
+    """The 4M-atom sample's output stage (Code/main.cpp:797-852): final positions
+    (x + u wrapped into the box, shifted by +L/2) and both files, extended XYZ and
+    the LAMMPS data file, from the device displacements of the FP32C sum; against
+    the reference's host writers (the oracle's iostream restatement, one host
+    thread, as the reference writes) on a bounded atom sample."""
+    import shutil
+    import tempfile
+    import torch
+    import oracle
+    from . import backend as B
+    from . import geometry as G
+    from . import writers as W
+    s = G.config_c2()
+    ctx = B.Context(0)
+    dev = torch.device("cuda:0")
+    x, y, z = (torch.from_numpy(a).to(dev) for a in (s.x, s.y, s.z))
+    u = [torch.zeros_like(x) for _ in range(3)]
+    plan = B.Plan(ctx, s.loops, s.box, s.img, B.PREC_FP32C)
+    plan.run_torch(x, y, z, *u)
+    torch.cuda.synchronize()
+    plan.free()
+    box = np.asarray(s.box, dtype=np.float64)
+    Hp = np.diag(box)
+    n = s.n
+    tmp = tempfile.mkdtemp(prefix="disco_out_")
+    name = os.path.join(tmp, "sample.xyz")
+    try:
+        # device kernels alone: final positions, then the text of both formats in HBM
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        f = W.final_positions_torch(ctx, x, y, z, *u, box)
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(args.steps):
+            W.final_positions_torch(ctx, x, y, z, *u, box, out=f)
+        ev[1].record()
+        torch.cuda.synchronize()
+        fp_ms = ev[0].elapsed_time(ev[1]) / args.steps
+        W.text_profile(ctx, True)
+        _, frame = W.lammps_box(Hp)
+        kern = {}
+        for fmt in ("xyz", "lammps"):
+            t = W.format_text_torch(ctx, fmt, *f, element="Cu", frame=frame, host=False)
+            ms = []
+            for _ in range(args.steps):
+                W.format_text_torch(ctx, fmt, *f, element="Cu", frame=frame, host=False)
+                ms.append(W.text_kernel_ms(ctx))
+            kern[fmt] = {"bytes": int(t.numel()),
+                         "count_ms": float(np.mean([m[0] for m in ms])),
+                         "scan_ms": float(np.mean([m[1] for m in ms])),
+                         "write_ms": float(np.mean([m[2] for m in ms]))}
+            del t
+        W.text_profile(ctx, False)
+        print(f"[writers] final positions {fp_ms:.3f} ms; text kernels (count/scan/write, ms): "
+              + "; ".join(f"{k} {v['count_ms']:.3f}/{v['scan_ms']:.3f}/{v['write_ms']:.3f} "
+                          f"({v['bytes'] / 1e6:.1f} MB)" for k, v in kern.items()), flush=True)
+
+        # whole output step, files included
+        for _ in range(max(1, args.warmup)):
+            W.write_outputs(ctx, name, "both", x, y, z, *u, box, "Cu", 63.546)
+        torch.cuda.synchronize()
+        walls = []
+        for k in range(args.steps):
+            t0 = time.perf_counter()
+            W.write_outputs(ctx, name, "both", x, y, z, *u, box, "Cu", 63.546)
+            torch.cuda.synchronize()
+            walls.append(time.perf_counter() - t0)
+            print(f"[writers] step {k + 1}/{args.steps}: {walls[-1] * 1e3:.1f} ms", flush=True)
+        wall = float(np.mean(walls))
+        nx, nl = os.path.getsize(name), os.path.getsize(name[:-3] + "lmp")
+
+        # parity on this workload: the first 200,000 atoms, against the oracle
+        fh = [t.cpu().numpy() for t in f]
+        uh = [t.cpu().numpy() for t in u]
+        m = 200_000
+        want = oracle.final_positions(s.x[:m], s.y[:m], s.z[:m], *(a[:m] for a in uh), box)
+        exact_pos = all(np.array_equal(a[:m], b) for a, b in zip(fh, want))
+        oracle.write_xyz(os.path.join(tmp, "o.xyz"), *want, Hp, "Cu")
+        with open(name, "rb") as g, open(os.path.join(tmp, "o.xyz"), "rb") as o:
+            g.readline()
+            o.readline()
+            ref_bytes = o.read()                      # Lattice line + the first m atom lines
+            exact_txt = g.read(len(ref_bytes)) == ref_bytes
+        cpu = None
+        if not args.no_cpu:
+            ms_ = n // 8
+            t0 = time.perf_counter()
+            fo = oracle.final_positions(s.x[:ms_], s.y[:ms_], s.z[:ms_], *(a[:ms_] for a in uh),
+                                        box)
+            oracle.write_xyz(os.path.join(tmp, "c.xyz"), *fo, Hp, "Cu")
+            oracle.write_lammps(os.path.join(tmp, "c.lmp"), *fo, Hp, "Cu", 63.546)
+            dt = time.perf_counter() - t0
+            cpu = {"value": ms_ / dt, "unit": "atoms/s", "cores": 1, "kind": "port",
+                   "sample": f"the first {ms_} atoms (one eighth), final positions + both files "
+                             f"through iostreams on one host thread: {dt:.2f} s",
+                   "seconds_per_sample_scaled": dt * n / ms_}
+            print(f"[writers] host writers: {dt:.2f} s for {ms_} atoms "
+                  f"(x8 = {dt * 8:.1f} s per sample)", flush=True)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    wk = kern["xyz"]
+    gbs = (24.0 * n + wk["bytes"]) / (wk["write_ms"] * 1e-3) / 1e9
+    _emit({"metric": "atoms/s written (final positions + XYZ + LAMMPS files)",
+           "value": n / wall, "unit": "atoms/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64 (text: fixed, 8 decimals)",
+           "data": "synthetic",
+           "config": {"workload": "FCC a0=3.615 100^3 (4,000,000 atoms) + its FP32C "
+                                  "displacements: final positions, .xyz and .lmp files",
+                      "xyz_bytes": nx, "lmp_bytes": nl, "final_positions_ms": fp_ms,
+                      "text_kernels": kern, "bit_exact_positions_first_200k": exact_pos,
+                      "byte_exact_xyz_first_200k": exact_txt,
+                      "files_per_s_GBps": (nx + nl) / wall / 1e9},
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s",
+                        "frac": gbs / 8000.0, "traffic": None,
+                        "kernel": "text_write_kernel<DISCO_TEXT_XYZ>",
+                        "kernel_ms": wk["write_ms"],
+                        "bytes_per_launch": 24.0 * n + wk["bytes"]},
+           "cpu_baseline": cpu})
+    ctx.close()
+
+
 def run(args):
     if not hasattr(args, "train_loops"):
         args.train_loops = 2000
     if not hasattr(args, "epochs"):
         args.epochs = 30
-    {"c3": run_c3, "c4": run_c4, "c5": run_c5, "lattice": run_lattice}[args.workload](args)
+    {"c3": run_c3, "c4": run_c4, "c5": run_c5, "lattice": run_lattice,
+     "writers": run_writers}[args.workload](args)

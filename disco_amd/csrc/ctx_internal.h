@@ -23,6 +23,10 @@ struct disco_ctx {
     char* d_text = nullptr;
     size_t cap_text = 0;
     char* h_stage[2] = {nullptr, nullptr};
+    // optional timing of the text kernels of the last format call (count, scan, write)
+    int text_profile = 0;
+    hipEvent_t text_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    float text_ms[3] = {0.f, 0.f, 0.f};
 };
 
 namespace disco {

@@ -445,6 +445,8 @@ int format_lines(disco_ctx* ctx, int format, const TextArgs& a, char* d_out, lon
     // (this path synchronizes below, so it is never inside a graph capture)
     DISCO_HIP_TRY(ctx, hipMemsetAsync(sc, 0, 16, s));
     const dim3 grid((unsigned)nb);
+    const bool prof = ctx->text_profile != 0;
+    if (prof) DISCO_HIP_TRY(ctx, hipEventRecord(ctx->text_ev[0], s));
     if (format == DISCO_TEXT_XYZ)
         hipLaunchKernelGGL(text_count_kernel<DISCO_TEXT_XYZ>, grid, dim3(kTextBlock), 0, s, a,
                            block_bytes, flags);
@@ -452,17 +454,25 @@ int format_lines(disco_ctx* ctx, int format, const TextArgs& a, char* d_out, lon
         hipLaunchKernelGGL(text_count_kernel<DISCO_TEXT_LAMMPS>, grid, dim3(kTextBlock), 0, s, a,
                            block_bytes, flags);
     DISCO_HIP_TRY(ctx, hipGetLastError());
+    if (prof) DISCO_HIP_TRY(ctx, hipEventRecord(ctx->text_ev[1], s));
     hipLaunchKernelGGL(text_scan_kernel, dim3(1), dim3(kScanBlock), 0, s, block_bytes, block_off,
                        (int)nb, total);
     DISCO_HIP_TRY(ctx, hipGetLastError());
+    if (prof) DISCO_HIP_TRY(ctx, hipEventRecord(ctx->text_ev[2], s));
     unsigned long long host[2];
     DISCO_HIP_TRY(ctx, hipMemcpyAsync(host, sc, 16, hipMemcpyDeviceToHost, s));
     DISCO_HIP_TRY(ctx, hipStreamSynchronize(s));
     if (host[0] & 1u)
         return fail(ctx, DISCO_ERR_TOO_LARGE, "a coordinate to print has |v| >= 1e10");
     *nbytes = (long long)host[1];
+    if (prof) {
+        DISCO_HIP_TRY(ctx, hipEventElapsedTime(&ctx->text_ms[0], ctx->text_ev[0], ctx->text_ev[1]));
+        DISCO_HIP_TRY(ctx, hipEventElapsedTime(&ctx->text_ms[1], ctx->text_ev[1], ctx->text_ev[2]));
+        ctx->text_ms[2] = 0.f;
+    }
     if (!d_out) return DISCO_OK;
     if (*nbytes > cap) return fail(ctx, DISCO_ERR_TOO_LARGE, "text buffer too small");
+    if (prof) DISCO_HIP_TRY(ctx, hipEventRecord(ctx->text_ev[2], s));
     if (format == DISCO_TEXT_XYZ)
         hipLaunchKernelGGL(text_write_kernel<DISCO_TEXT_XYZ>, grid, dim3(kTextBlock), 0, s, a,
                            block_off, d_out);
@@ -470,7 +480,10 @@ int format_lines(disco_ctx* ctx, int format, const TextArgs& a, char* d_out, lon
         hipLaunchKernelGGL(text_write_kernel<DISCO_TEXT_LAMMPS>, grid, dim3(kTextBlock), 0, s, a,
                            block_off, d_out);
     DISCO_HIP_TRY(ctx, hipGetLastError());
+    if (prof) DISCO_HIP_TRY(ctx, hipEventRecord(ctx->text_ev[3], s));
     DISCO_HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (prof)
+        DISCO_HIP_TRY(ctx, hipEventElapsedTime(&ctx->text_ms[2], ctx->text_ev[2], ctx->text_ev[3]));
     return DISCO_OK;
 }
 
@@ -590,6 +603,9 @@ void release_output_buffers(disco_ctx* ctx)
     ctx->d_text = nullptr;
     ctx->h_stage[0] = ctx->h_stage[1] = nullptr;
     ctx->cap_text_scratch = ctx->cap_text = 0;
+    for (int i = 0; i < 4; i++)
+        if (ctx->text_ev[i]) (void)hipEventDestroy(ctx->text_ev[i]);
+    for (int i = 0; i < 4; i++) ctx->text_ev[i] = nullptr;
 }
 
 }  // namespace disco
@@ -641,6 +657,25 @@ int disco_hip_format_text(disco_ctx* ctx, const disco_text_format* fmt, long lon
     if (rc) return rc;
     DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
     return format_lines(ctx, fmt->format, a, d_out, cap, nbytes, (hipStream_t)stream);
+}
+
+int disco_hip_text_profile(disco_ctx* ctx, int on)
+{
+    if (!ctx) return DISCO_ERR_ARG;
+    if (on && !ctx->text_ev[0]) {
+        DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+        for (int i = 0; i < 4; i++) DISCO_HIP_TRY(ctx, hipEventCreate(&ctx->text_ev[i]));
+    }
+    ctx->text_profile = on ? 1 : 0;
+    return DISCO_OK;
+}
+
+int disco_hip_text_kernel_ms(disco_ctx* ctx, double ms[3])
+{
+    if (!ctx || !ms) return DISCO_ERR_ARG;
+    if (!ctx->text_profile) return fail(ctx, DISCO_ERR_STATE, "text profiling is off");
+    for (int i = 0; i < 3; i++) ms[i] = ctx->text_ms[i];
+    return DISCO_OK;
 }
 
 int disco_hip_write_xyz(disco_ctx* ctx, const char* path, long long n, const double* d_fx,

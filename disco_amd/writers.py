@@ -43,6 +43,8 @@ def _lib():
                                           ctypes.c_char_p, _vp, _vp, _vp]
         L.disco_hip_write_lammps.argtypes = [_vp, ctypes.c_char_p, _ll, _vp, _vp, _vp, _dp,
                                              ctypes.c_char_p, ctypes.c_double, _vp]
+        L.disco_hip_text_profile.argtypes = [_vp, ctypes.c_int]
+        L.disco_hip_text_kernel_ms.argtypes = [_vp, _dp]
         _bound = True
     return L
 
@@ -118,6 +120,18 @@ def format_text_torch(ctx, fmt, fx, fy, fz, element="Fe", grain=None, twin=None,
                                       _stream()))
     buf = buf[:nbytes.value]
     return buf.cpu().numpy().tobytes() if host else buf
+
+
+def text_profile(ctx, on=True):
+    """Record the durations of the text kernels of every format call."""
+    ctx.check(_lib().disco_hip_text_profile(ctx._h, 1 if on else 0))
+
+
+def text_kernel_ms(ctx):
+    """(count, scan, write) kernel ms of the last format call (profiling on)."""
+    ms = np.zeros(3)
+    ctx.check(_lib().disco_hip_text_kernel_ms(ctx._h, ms.ctypes.data_as(_dp)))
+    return tuple(float(v) for v in ms)
 
 
 def write_xyz(ctx, path, fx, fy, fz, Hp, element, grain=None, twin=None):

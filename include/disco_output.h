@@ -66,6 +66,12 @@ int disco_hip_format_text(disco_ctx *ctx, const disco_text_format *fmt, long lon
                           const double *d_fx, const double *d_fy, const double *d_fz,
                           char *d_out, long long cap, long long *nbytes, void *stream);
 
+/* Timing of the text kernels (HIP events on the call's stream): with profiling
+ * on, every format call (also inside the file writers) records the durations of
+ * its last count, scan and write kernels, read back in ms[0..2]. */
+int disco_hip_text_profile(disco_ctx *ctx, int on);
+int disco_hip_text_kernel_ms(disco_ctx *ctx, double ms[3]);
+
 /* Whole files from device final positions.  The XYZ header lists Hp's columns
  * as the Lattice string; grain / twin columns are written when given.  The
  * LAMMPS file is the atomic-style data file of writeLAMMPS (tilt line only
