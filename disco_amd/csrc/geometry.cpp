@@ -1,11 +1,29 @@
 // geometry.cpp -- host producer of loop inputs (include/disco_geometry.h).
 //
 // Follows the reference's loop code formula by formula and in its operation
-// order (every product and sum rounded on its own: contraction is off in this
-// file), so the polylines, loopon grids and snapped centres equal the
-// reference's: regularPolygon (Code/loop.cpp:585-608), defaultAxes (:40-57),
-// snapToMidPlane (:342-374) with latticePlaneSpacing (Code/lattice.hpp:146-153),
-// loop::polyline (:536-580) and generateLoopons / insideUV (:253-316).
+// order: regularPolygon (Code/loop.cpp:585-608), initCommon / defaultAxes
+// (:13-57), the loop constructors (:62-244), snapToMidPlane (:342-374) with
+// latticePlaneSpacing (Code/lattice.hpp:146-153), loop::polyline and
+// roundedPolygon2D (:478-580), generateLoopons / insideUV (:253-316) and
+// generateSurface (:426-435).
+//
+// Rounding: the reference's host build (g++ -O3 -march=native) contracts
+// products into fused multiply-adds, so the same formula rounds differently
+// with and without FMA hardware.  This file has contraction off and writes
+// every fused operation the reference's build performs as an explicit
+// std::fma, by GCC's rule for a chain of sums of products: the first product
+// of a chain is fused with what it is added to, later products with the
+// running sum --
+//   a*b + c*d        -> fma(a, b, c*d)
+//   a*b + c*d + e*f  -> fma(e, f, fma(a, b, c*d))
+//   a*b - c*d        -> fma(a, b, -(c*d))
+//   x - a*b          -> fma(-a, b, x),   x + a*b -> fma(a, b, x)
+// The same build evaluates sin(x) and cos(x) of one argument as one sincos(x)
+// call, whose last bit can differ from sin's or cos's (glibc); those pairs call
+// sincos here too.  So the results are bit-identical to the compiled
+// reference's (tested against the reference's own sources compiled with its
+// flags).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -23,33 +41,32 @@ struct V3 {
     double operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
 };
 
-double dot(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+V3 v3(const double* p) { return {p[0], p[1], p[2]}; }
+double dot3(const V3& a, const V3& b) { return std::fma(a.z, b.z, std::fma(a.x, b.x, a.y * b.y)); }
 V3 cross(const V3& a, const V3& b)
 {
-    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+    return {std::fma(a.y, b.z, -(a.z * b.y)), std::fma(a.z, b.x, -(a.x * b.z)),
+            std::fma(a.x, b.y, -(a.y * b.x))};
 }
-V3 unit(const V3& v)
-{
-    const double l = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
-    return {v.x / l, v.y / l, v.z / l};
-}
-V3 v3(const double* p) { return {p[0], p[1], p[2]}; }
+V3 scale_inv(const V3& a, double l) { return {a.x / l, a.y / l, a.z / l}; }
+double mx(double a, double b) { return a < b ? b : a; }   // std::max
+double mn(double a, double b) { return b < a ? b : a; }   // std::min
 
 // in-plane axes (u, v) for unit m, n (Code/loop.cpp:40-57)
 void default_axes(const V3& m, const V3& n, V3& u, V3& v)
 {
-    const double mn = m.x * n.x + m.y * n.y + m.z * n.z;
-    u = {m.x - mn * n.x, m.y - mn * n.y, m.z - mn * n.z};
-    double l = std::sqrt(u.x * u.x + u.y * u.y + u.z * u.z);
+    const double mn_ = dot3(m, n);
+    u = {std::fma(-mn_, n.x, m.x), std::fma(-mn_, n.y, m.y), std::fma(-mn_, n.z, m.z)};
+    double l = std::sqrt(dot3(u, u));
     if (l < 1e-6) {
         int k = 0;
         for (int i = 1; i < 3; i++)
             if (std::fabs(n[i]) < std::fabs(n[k])) k = i;
         const V3 e = {k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
         u = cross(n, e);
-        l = std::sqrt(u.x * u.x + u.y * u.y + u.z * u.z);
+        l = std::sqrt(dot3(u, u));
     }
-    u = {u.x / l, u.y / l, u.z / l};
+    u = scale_inv(u, l);
     v = cross(u, n);
 }
 
@@ -64,102 +81,239 @@ double plane_spacing(int lattice, const long h[3], double a0)
     return full ? a0 / norm : a0 / (2.0 * norm);
 }
 
-// A loop prepared as the reference prepares it: unit m and n, |b|, the axes,
-// and for a polygon its in-plane coordinates, counter-clockwise.
+// A loop as the reference's constructors leave it.
 struct Loop {
     int shape = 0;
     V3 c{}, mh{}, nh{}, u{}, v{};
-    double bmag = 0, radius = 0, half_m = 0, half_p = 0;
-    std::vector<double> pu, pv;
+    double bmag = 0, radius = 0, half_m = 0, half_p = 0, r_corner = 0;
+    std::vector<double> pu, pv, qu, qv, nu, nv;
 };
+
+bool has_axis(const double* a) { return a[0] != 0.0 || a[1] != 0.0 || a[2] != 0.0; }
+
+// Polygon part of the constructor (Code/loop.cpp:178-244): vertices on (u, v),
+// counter-clockwise, strict convexity, corner clamp, inward normals, inner
+// polygon Q.
+bool prepare_polygon(const disco_loop_shape* s, Loop& L)
+{
+    const int N = s->nvert;
+    if (N < 3 || !s->vert) return false;
+    for (int i = 0; i < N; i++) {
+        const V3 x = v3(s->vert + 3 * i);
+        L.pu.push_back(dot3(x, L.u));
+        L.pv.push_back(dot3(x, L.v));
+    }
+    double area2 = 0.0;
+    for (int i = 0; i < N; i++) {
+        const int j = (i + 1) % N;
+        area2 += std::fma(L.pu[i], L.pv[j], -(L.pu[j] * L.pv[i]));
+    }
+    if (std::fabs(area2) < 1e-12) return false;
+    if (area2 < 0.0) {
+        std::reverse(L.pu.begin(), L.pu.end());
+        std::reverse(L.pv.begin(), L.pv.end());
+    }
+    std::vector<double> du(N), dv(N), len(N), cot_half(N);
+    for (int i = 0; i < N; i++) {
+        const int j = (i + 1) % N;
+        const double eu = L.pu[j] - L.pu[i], ev = L.pv[j] - L.pv[i];
+        len[i] = std::sqrt(std::fma(eu, eu, ev * ev));
+        if (len[i] < 1e-9) return false;
+        du[i] = eu / len[i];
+        dv[i] = ev / len[i];
+    }
+    double turn = 0.0;
+    for (int i = 0; i < N; i++) {
+        const int k = (i + N - 1) % N;
+        const double sn = std::fma(du[k], dv[i], -(dv[k] * du[i]));
+        const double cs = std::fma(du[k], du[i], dv[k] * dv[i]);
+        if (sn < 1e-9) return false;                       // not strictly convex
+        turn += std::atan2(sn, cs);
+        cot_half[i] = (1.0 - cs) / sn;
+    }
+    if (std::fabs(turn - 2.0 * kPiGeo) > 1e-6) return false;   // self-intersecting
+    double r_max = 1e300;
+    for (int i = 0; i < N; i++) r_max = mn(r_max, len[i] / (cot_half[i] + cot_half[(i + 1) % N]));
+    double r = s->corner_r;
+    if (r < 0.0) return false;
+    if (r > r_max) r = r_max;
+    L.r_corner = r;
+    L.nu.resize(N);
+    L.nv.resize(N);
+    for (int i = 0; i < N; i++) {
+        L.nu[i] = -dv[i];
+        L.nv[i] = du[i];
+    }
+    L.qu.resize(N);
+    L.qv.resize(N);
+    for (int i = 0; i < N; i++) {
+        const int k = (i + N - 1) % N;
+        const double n1u = -dv[k], n1v = du[k], n2u = -dv[i], n2v = du[i];
+        const double f = L.r_corner / std::fma(n1v, n2v, std::fma(n1u, n2u, 1.0));
+        L.qu[i] = std::fma(f, n1u + n2u, L.pu[i]);
+        L.qv[i] = std::fma(f, n1v + n2v, L.pv[i]);
+    }
+    for (int i = 0; i < N; i++) {
+        L.half_m = mx(L.half_m, std::fabs(L.pu[i]));
+        L.half_p = mx(L.half_p, std::fabs(L.pv[i]));
+    }
+    return true;
+}
 
 bool prepare(const disco_loop_shape* s, Loop& L)
 {
     if (!s || s->shape < DISCO_SHAPE_CIRCLE || s->shape > DISCO_SHAPE_PARALLELOGRAM) return false;
     L.shape = s->shape;
     L.c = v3(s->center);
-    const double normm = std::sqrt(s->m[0] * s->m[0] + s->m[1] * s->m[1] + s->m[2] * s->m[2]);
-    if (!(normm > 0.0)) return false;
-    L.mh = {s->m[0] / normm, s->m[1] / normm, s->m[2] / normm};
-    const V3 n = v3(s->n);
-    if (!(dot(n, n) > 0.0)) return false;
-    L.nh = unit(n);
-    L.bmag = s->a0 * 0.5 * normm;
-    L.radius = s->radius;
-    L.half_m = s->half_m;
-    L.half_p = s->half_p;
-    if (s->shape == DISCO_SHAPE_PARALLELOGRAM) {
-        L.u = unit(v3(s->dir1));
-        L.v = unit(v3(s->dir2));
-    } else {
-        default_axes(L.mh, L.nh, L.u, L.v);
+    // initCommon (Code/loop.cpp:13-35)
+    const V3 m = v3(s->m), n = v3(s->n);
+    const double norm_m = std::sqrt(dot3(m, m));
+    const double norm_n = std::sqrt(dot3(n, n));
+    if (!(norm_m > 0.0) || !(norm_n > 0.0)) return false;
+    L.mh = scale_inv(m, norm_m);
+    L.nh = scale_inv(n, norm_n);
+    L.bmag = 1.0 * s->a0 * 0.5 * norm_m;
+    switch (s->shape) {
+        case DISCO_SHAPE_CIRCLE:
+            if (!(s->radius > 0.0)) return false;
+            L.radius = s->radius;
+            default_axes(L.mh, L.nh, L.u, L.v);
+            return true;
+        case DISCO_SHAPE_RECT: {
+            L.half_m = s->half_m;
+            L.half_p = s->half_p;
+            if (has_axis(s->axis)) {
+                const V3 sd = v3(s->axis);
+                L.u = scale_inv(sd, std::sqrt(dot3(sd, sd)));
+                L.v = cross(L.u, L.nh);
+            } else {
+                default_axes(L.mh, L.nh, L.u, L.v);
+            }
+            const double max_corner = mn(L.half_m, L.half_p);
+            L.r_corner = s->corner_r > max_corner ? max_corner : s->corner_r;
+            return true;
+        }
+        case DISCO_SHAPE_PARALLELOGRAM: {
+            L.half_m = s->half_m;
+            L.half_p = s->half_p;
+            const V3 d1 = v3(s->dir1), d2 = v3(s->dir2);
+            L.u = scale_inv(d1, std::sqrt(dot3(d1, d1)));
+            L.v = scale_inv(d2, std::sqrt(dot3(d2, d2)));
+            return true;
+        }
+        default:
+            if (has_axis(s->axis)) {
+                const V3 a = v3(s->axis);
+                const double an = dot3(a, L.nh);
+                const V3 w = {std::fma(-an, L.nh.x, a.x), std::fma(-an, L.nh.y, a.y),
+                              std::fma(-an, L.nh.z, a.z)};
+                L.u = scale_inv(w, std::sqrt(dot3(w, w)));
+                L.v = cross(L.u, L.nh);
+            } else {
+                default_axes(L.mh, L.nh, L.u, L.v);
+            }
+            return prepare_polygon(s, L);
     }
-    if (s->shape == DISCO_SHAPE_CIRCLE && !(s->radius > 0.0)) return false;
-    if (s->shape == DISCO_SHAPE_POLYGON) {
-        if (s->nvert < 3 || !s->vert) return false;
-        const int N = s->nvert;
-        L.pu.resize(N);
-        L.pv.resize(N);
-        for (int i = 0; i < N; i++) {
-            const V3 x = v3(s->vert + 3 * i);
-            L.pu[i] = x.x * L.u.x + x.y * L.u.y + x.z * L.u.z;
-            L.pv[i] = x.x * L.v.x + x.y * L.v.y + x.z * L.v.z;
-        }
-        double a2 = 0.0;
-        for (int i = 0; i < N; i++)
-            a2 += L.pu[i] * L.pv[(i + 1) % N] - L.pu[(i + 1) % N] * L.pv[i];
-        if (a2 < 0.0) {
-            std::vector<double> ru(L.pu.rbegin(), L.pu.rend()), rv(L.pv.rbegin(), L.pv.rend());
-            L.pu.swap(ru);
-            L.pv.swap(rv);
-        }
-        double hm = 0.0, hp = 0.0;
-        for (int i = 0; i < N; i++) {
-            hm = std::fmax(hm, std::fabs(L.pu[i]));
-            hp = std::fmax(hp, std::fabs(L.pv[i]));
-        }
-        L.half_m = hm;
-        L.half_p = hp;
-    }
-    return true;
 }
 
-// contour in the plane coordinates (a along u, b along v)
+// chords for an arc of angle th, radius r, sagitta <= eps (Code/loop.cpp:478-484)
+int arc_chords(double th, double r, double eps)
+{
+    if (r <= 0.0 || th <= 0.0) return 0;
+    const double c = 1.0 - eps / r;
+    const double step = c <= -1.0 ? kPiGeo : 2.0 * std::acos(mx(c, -1.0));
+    const int k = (int)std::ceil(th / step - 1e-12);
+    return k > 1 ? k : 1;
+}
+
+// Code/loop.cpp:489-527: the 2D contour of a (rounded) convex polygon, CCW
+void rounded_polygon(const std::vector<double>& pu, const std::vector<double>& pv, double r,
+                     double eps, std::vector<double>& ou, std::vector<double>& ov)
+{
+    const int N = (int)pu.size();
+    auto push = [&](double a, double b) {
+        if (!ou.empty() && std::fabs(ou.back() - a) < 1e-12 && std::fabs(ov.back() - b) < 1e-12)
+            return;
+        ou.push_back(a);
+        ov.push_back(b);
+    };
+    if (r <= 0.0) {
+        for (int i = 0; i < N; i++) push(pu[i], pv[i]);
+    } else {
+        std::vector<double> nu(N), nv(N);   // outward unit normals
+        for (int i = 0; i < N; i++) {
+            const int j = (i + 1) % N;
+            const double eu = pu[j] - pu[i], ev = pv[j] - pv[i];
+            const double l = std::sqrt(std::fma(eu, eu, ev * ev));
+            nu[i] = ev / l;
+            nv[i] = -eu / l;
+        }
+        for (int i = 0; i < N; i++) {
+            const int k = (i + N - 1) % N;
+            const double f = r / std::fma(nv[k], nv[i], std::fma(nu[k], nu[i], 1.0));
+            const double qu = std::fma(-f, nu[k] + nu[i], pu[i]);
+            const double qv = std::fma(-f, nv[k] + nv[i], pv[i]);
+            const double a0 = std::atan2(nv[k], nu[k]);
+            double th = std::atan2(nv[i], nu[i]) - a0;
+            while (th < 0.0) th += 2.0 * kPiGeo;
+            while (th >= 2.0 * kPiGeo) th -= 2.0 * kPiGeo;
+            const int nc = arc_chords(th, r, eps);
+            for (int c = 0; c <= nc; c++) {
+                const double a = a0 + th * c / (nc > 1 ? nc : 1);
+                double sa, ca;
+                ::sincos(a, &sa, &ca);
+                push(std::fma(r, ca, qu), std::fma(r, sa, qv));
+            }
+        }
+    }
+    if (ou.size() > 1 && std::fabs(ou.back() - ou[0]) < 1e-12 &&
+        std::fabs(ov.back() - ov[0]) < 1e-12) {
+        ou.pop_back();
+        ov.pop_back();
+    }
+}
+
+// contour in plane coordinates (a along u, b along v); nseg > 0 forces the
+// chord count of a circle
 void plane_coords(const Loop& L, int nseg, double eps, std::vector<double>& a,
                   std::vector<double>& b)
 {
     a.clear();
     b.clear();
     if (L.shape == DISCO_SHAPE_CIRCLE) {
-        if (nseg <= 0) {
-            const double c = 1.0 - eps / L.radius;
-            const double step = c <= -1.0 ? kPiGeo : 2.0 * std::acos(std::fmax(c, -1.0));
-            const int n = (int)std::ceil(2.0 * kPiGeo / step - 1e-12);
-            nseg = n > 8 ? n : 8;
+        int n = nseg;
+        if (n <= 0) {
+            const int k = arc_chords(2.0 * kPiGeo, L.radius, eps);
+            n = k > 8 ? k : 8;
         }
-        const double rp = L.radius * std::sqrt(2.0 * kPiGeo / (nseg * std::sin(2.0 * kPiGeo / nseg)));
-        for (int k = 0; k < nseg; k++) {
-            a.push_back(rp * std::cos(2.0 * kPiGeo * k / nseg));
-            b.push_back(rp * std::sin(2.0 * kPiGeo * k / nseg));
+        const double rp = L.radius * std::sqrt(2.0 * kPiGeo / (n * std::sin(2.0 * kPiGeo / n)));
+        for (int k = 0; k < n; k++) {
+            double st, ct;
+            ::sincos(2.0 * kPiGeo * k / n, &st, &ct);
+            a.push_back(rp * ct);
+            b.push_back(rp * st);
         }
         return;
     }
     if (L.shape == DISCO_SHAPE_RECT || L.shape == DISCO_SHAPE_PARALLELOGRAM) {
-        a = {L.half_m, -L.half_m, -L.half_m, L.half_m};
-        b = {L.half_p, L.half_p, -L.half_p, -L.half_p};
+        const std::vector<double> pu = {L.half_m, -L.half_m, -L.half_m, L.half_m};
+        const std::vector<double> pv = {L.half_p, L.half_p, -L.half_p, -L.half_p};
+        if (L.shape == DISCO_SHAPE_PARALLELOGRAM) {
+            a = pu;
+            b = pv;
+            return;
+        }
+        rounded_polygon(pu, pv, L.r_corner, eps, a, b);
         return;
     }
-    for (size_t i = 0; i < L.pu.size(); i++) {
-        if (!a.empty() && std::fabs(a.back() - L.pu[i]) < 1e-12 &&
-            std::fabs(b.back() - L.pv[i]) < 1e-12)
-            continue;
-        a.push_back(L.pu[i]);
-        b.push_back(L.pv[i]);
-    }
-    if (a.size() > 1 && std::fabs(a.back() - a[0]) < 1e-12 && std::fabs(b.back() - b[0]) < 1e-12) {
-        a.pop_back();
-        b.pop_back();
-    }
+    rounded_polygon(L.pu, L.pv, L.r_corner, eps, a, b);
+}
+
+// in-plane point to space: u a + v b + c (Code/loop.cpp:564-566, compress = 1)
+V3 to_space(const Loop& L, double a, double b)
+{
+    return {std::fma(L.u.x, a, L.v.x * b) + L.c.x, std::fma(L.u.y, a, L.v.y * b) + L.c.y,
+            std::fma(L.u.z, a, L.v.z * b) + L.c.z};
 }
 
 std::vector<V3> polyline(const Loop& L, double eps, int nseg)
@@ -168,22 +322,22 @@ std::vector<V3> polyline(const Loop& L, double eps, int nseg)
     plane_coords(L, nseg, eps, a, b);
     const size_t N = a.size();
     std::vector<V3> out(N);
-    for (size_t k = 0; k < N; k++)
-        out[k] = {(L.u.x * a[k] + L.v.x * b[k]) + L.c.x, (L.u.y * a[k] + L.v.y * b[k]) + L.c.y,
-                  (L.u.z * a[k] + L.v.z * b[k]) + L.c.z};
+    for (size_t k = 0; k < N; k++) out[k] = to_space(L, a[k], b[k]);
     double s = 0.0;
     for (size_t k = 0; k < N; k++) {
         const V3& p = out[k];
         const V3& q = out[(k + 1) % N];
-        const V3 cr = cross({p.x - L.c.x, p.y - L.c.y, p.z - L.c.z},
-                            {q.x - L.c.x, q.y - L.c.y, q.z - L.c.z});
-        s += cr.x * L.nh.x + cr.y * L.nh.y + cr.z * L.nh.z;
+        const V3 r1 = {p.x - L.c.x, p.y - L.c.y, p.z - L.c.z};
+        const V3 r2 = {q.x - L.c.x, q.y - L.c.y, q.z - L.c.z};
+        const V3 cr = cross(r1, r2);
+        s += std::fma(cr.z, L.nh.z, std::fma(cr.x, L.nh.x, cr.y * L.nh.y));
     }
-    if (s < 0.0) std::vector<V3>(out.rbegin(), out.rend()).swap(out);
+    if (s < 0.0) std::reverse(out.begin(), out.end());
     return out;
 }
 
 // sharp contour with each side cut into collinear pieces, nseg_total in all
+// (a workload construct of this producer, not of the reference)
 std::vector<V3> subdivided(const Loop& L, int nseg_total)
 {
     const std::vector<V3> P = polyline(L, 0.01, 0);
@@ -200,24 +354,35 @@ std::vector<V3> subdivided(const Loop& L, int nseg_total)
     return out;
 }
 
+// Code/loop.cpp:281-316
 bool inside_uv(const Loop& L, double u, double v)
 {
     const double eps = 1e-9;
-    if (L.shape == DISCO_SHAPE_CIRCLE) return std::sqrt(u * u + v * v) < L.radius - eps;
-    if (L.shape == DISCO_SHAPE_RECT || L.shape == DISCO_SHAPE_PARALLELOGRAM) {
-        const double du = std::fmax(std::fabs(u) - std::fmax(L.half_m, 0.0), 0.0);
-        const double dv = std::fmax(std::fabs(v) - std::fmax(L.half_p, 0.0), 0.0);
-        return du * du + dv * dv <= eps;
+    if (L.shape == DISCO_SHAPE_CIRCLE) return std::sqrt(std::fma(u, u, v * v)) < L.radius - eps;
+    if (L.shape == DISCO_SHAPE_POLYGON) {
+        const size_t N = L.qu.size();
+        bool in_q = true;
+        for (size_t i = 0; i < N && in_q; i++)
+            if (std::fma(L.nu[i], u - L.pu[i], L.nv[i] * (v - L.pv[i])) < L.r_corner - eps)
+                in_q = false;
+        if (in_q) return true;
+        if (L.r_corner <= 0.0) return false;
+        double d2 = 1e300;
+        for (size_t i = 0; i < N; i++) {
+            const size_t j = (i + 1) % N;
+            const double eu = L.qu[j] - L.qu[i], ev = L.qv[j] - L.qv[i];
+            const double l2 = std::fma(eu, eu, ev * ev);
+            double t = l2 > 0.0 ? std::fma(u - L.qu[i], eu, (v - L.qv[i]) * ev) / l2 : 0.0;
+            t = mn(mx(t, 0.0), 1.0);
+            const double wu = std::fma(-t, eu, u - L.qu[i]);
+            const double wv = std::fma(-t, ev, v - L.qv[i]);
+            d2 = mn(d2, std::fma(wu, wu, wv * wv));
+        }
+        return d2 <= std::fma(L.r_corner, L.r_corner, eps);
     }
-    const size_t N = L.pu.size();
-    for (size_t i = 0; i < N; i++) {
-        const size_t j = (i + 1) % N;
-        const double eu = L.pu[j] - L.pu[i], ev = L.pv[j] - L.pv[i];
-        const double len = std::sqrt(eu * eu + ev * ev);
-        const double nu = -ev / len, nv = eu / len;
-        if (!(nu * (u - L.pu[i]) + nv * (v - L.pv[i]) >= -eps)) return false;
-    }
-    return true;
+    const double du = mx(std::fabs(u) - mx(L.half_m - L.r_corner, 0.0), 0.0);
+    const double dv = mx(std::fabs(v) - mx(L.half_p - L.r_corner, 0.0), 0.0);
+    return std::fma(du, du, dv * dv) <= std::fma(L.r_corner, L.r_corner, eps);
 }
 
 double surface(const Loop& L, double da)
@@ -225,7 +390,7 @@ double surface(const Loop& L, double da)
     double ds = da * da;
     if (L.shape == DISCO_SHAPE_PARALLELOGRAM) {
         const V3 cr = cross(L.u, L.v);
-        ds *= std::sqrt(cr.x * cr.x + cr.y * cr.y + cr.z * cr.z);
+        ds *= std::sqrt(dot3(cr, cr));
     }
     return ds;
 }
@@ -239,6 +404,13 @@ int emit(const std::vector<V3>& pts, P* x, P* y, P* z, int cap)
     return n;
 }
 
+void put3(double* o, const V3& a)
+{
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = a.z;
+}
+
 }  // namespace
 
 extern "C" {
@@ -247,14 +419,17 @@ int disco_geo_regular_polygon(int sides, double R, const double m[3], const doub
                               double* out)
 {
     if (sides < 3 || !m || !n || !out) return DISCO_GEO_ERR_ARG;
+    const V3 mm = v3(m), nn = v3(n);
     V3 u, v;
-    default_axes(unit(v3(m)), unit(v3(n)), u, v);
+    default_axes(scale_inv(mm, std::sqrt(dot3(mm, mm))), scale_inv(nn, std::sqrt(dot3(nn, nn))),
+                 u, v);
     for (int k = 0; k < sides; k++) {
         const double th = 2.0 * kPiGeo * k / sides;
-        const double ct = std::cos(th), st = std::sin(th);
-        out[3 * k + 0] = R * (ct * u.x + st * v.x);
-        out[3 * k + 1] = R * (ct * u.y + st * v.y);
-        out[3 * k + 2] = R * (ct * u.z + st * v.z);
+        double st, ct;
+        ::sincos(th, &st, &ct);
+        out[3 * k + 0] = R * std::fma(ct, u.x, st * v.x);
+        out[3 * k + 1] = R * std::fma(ct, u.y, st * v.y);
+        out[3 * k + 2] = R * std::fma(ct, u.z, st * v.z);
     }
     return sides;
 }
@@ -279,12 +454,12 @@ int disco_geo_snap_mid_plane(const double c[3], const double n[3], double a0, in
     const V3 nh = {h[0] / norm, h[1] / norm, h[2] / norm};
     const double sp = plane_spacing(lattice, h, a0);
     const V3 o = {-0.5 * a0 * nx, -0.5 * a0 * ny, -0.5 * a0 * nz};
-    const double s0 = o.x * nh.x + o.y * nh.y + o.z * nh.z;
-    const double s = c[0] * nh.x + c[1] * nh.y + c[2] * nh.z;
-    const double s_mid = s0 + (std::floor((s - s0) / sp + 1e-9) + 0.5) * sp;
-    out[0] = c[0] + (s_mid - s) * nh.x;
-    out[1] = c[1] + (s_mid - s) * nh.y;
-    out[2] = c[2] + (s_mid - s) * nh.z;
+    const double s0 = dot3(o, nh);
+    const double s = dot3(v3(c), nh);
+    const double s_mid = std::fma(std::floor((s - s0) / sp + 1e-9) + 0.5, sp, s0);
+    out[0] = std::fma(s_mid - s, nh.x, c[0]);
+    out[1] = std::fma(s_mid - s, nh.y, c[1]);
+    out[2] = std::fma(s_mid - s, nh.z, c[2]);
     if (spacing) *spacing = sp;
     return 1;
 }
@@ -311,9 +486,7 @@ int disco_geo_loopons(const disco_loop_shape* s, double da, double* x, double* y
     for (int i = -sm; i <= sm; i++)
         for (int j = -sp; j <= sp; j++) {
             const double U = i * da, V = j * da;
-            if (!inside_uv(L, U, V)) continue;
-            pts.push_back({(L.u.x * U + L.v.x * V) + L.c.x, (L.u.y * U + L.v.y * V) + L.c.y,
-                           (L.u.z * U + L.v.z * V) + L.c.z});
+            if (inside_uv(L, U, V)) pts.push_back(to_space(L, U, V));
         }
     return emit(pts, x, y, z, cap);
 }
@@ -326,6 +499,35 @@ int disco_geo_phys(const disco_loop_shape* s, double da, double phys[10])
                           L.nh.x, L.nh.y, L.nh.z, da > 0 ? surface(L, da) : 0.0};
     std::memcpy(phys, p, sizeof(p));
     return 10;
+}
+
+int disco_geo_loop_state(const disco_loop_shape* s, disco_loop_state* out)
+{
+    Loop L;
+    if (!out || !prepare(s, L)) return DISCO_GEO_ERR_ARG;
+    if (L.pu.size() > DISCO_MAX_STATE_VERT) return DISCO_GEO_ERR_ARG;
+    std::memset(out, 0, sizeof(*out));
+    out->shape = L.shape;
+    out->nvert = (int)L.pu.size();
+    put3(out->center, L.c);
+    put3(out->m, L.mh);
+    put3(out->n, L.nh);
+    put3(out->u, L.u);
+    put3(out->v, L.v);
+    out->bmag = L.bmag;
+    out->radius = L.radius;
+    out->half_m = L.half_m;
+    out->half_p = L.half_p;
+    out->r_corner = L.r_corner;
+    for (size_t i = 0; i < L.pu.size(); i++) {
+        out->pu[i] = L.pu[i];
+        out->pv[i] = L.pv[i];
+        out->qu[i] = L.qu[i];
+        out->qv[i] = L.qv[i];
+        out->nu[i] = L.nu[i];
+        out->nv[i] = L.nv[i];
+    }
+    return 1;
 }
 
 long long disco_geo_polylines(const disco_loop_shape* shapes, int nloops, double eps,

@@ -152,6 +152,8 @@ class Loop:
     half_p: float = 0.0
     dir1: list = field(default_factory=list)
     dir2: list = field(default_factory=list)
+    corner_r: float = 0.0          # rect / polygon corner radius (native producer only)
+    axis: list = field(default_factory=list)   # rect Side / polygon grid axis (native only)
 
     def __post_init__(self):
         normm = math.sqrt(sum(t * t for t in self.m))
@@ -502,7 +504,20 @@ class _Shape(_ct.Structure):
                 ("n", _ct.c_double * 3), ("a0", _ct.c_double), ("radius", _ct.c_double),
                 ("half_m", _ct.c_double), ("half_p", _ct.c_double),
                 ("dir1", _ct.c_double * 3), ("dir2", _ct.c_double * 3), ("nvert", _ct.c_int),
-                ("vert", _ct.POINTER(_ct.c_double))]
+                ("vert", _ct.POINTER(_ct.c_double)), ("corner_r", _ct.c_double),
+                ("axis", _ct.c_double * 3)]
+
+
+_MAXV = 128
+
+
+class _LoopState(_ct.Structure):
+    """disco_loop_state: a loop as the reference's constructors leave it."""
+    _fields_ = [("shape", _ct.c_int), ("nvert", _ct.c_int), ("center", _ct.c_double * 3),
+                ("m", _ct.c_double * 3), ("n", _ct.c_double * 3), ("u", _ct.c_double * 3),
+                ("v", _ct.c_double * 3), ("bmag", _ct.c_double), ("radius", _ct.c_double),
+                ("half_m", _ct.c_double), ("half_p", _ct.c_double), ("r_corner", _ct.c_double)] + \
+               [(k, _ct.c_double * _MAXV) for k in ("pu", "pv", "qu", "qv", "nu", "nv")]
 
 
 _SHAPES = {"circle": 0, "polygon": 1, "rect": 2, "parallelogram": 3}
@@ -524,6 +539,7 @@ def _nlib():
                                           _ct.POINTER(i), _ct.POINTER(_ct.c_longlong), dp, dp,
                                           dp, _ct.c_longlong]
         L.disco_geo_polylines.restype = _ct.c_longlong
+        L.disco_geo_loop_state.argtypes = [_ct.POINTER(_Shape), _ct.POINTER(_LoopState)]
         _native = L
     return _native
 
@@ -543,6 +559,9 @@ def native_shape(lp: Loop):
     if lp.shape == "parallelogram":
         s.dir1[:] = [float(t) for t in lp.dir1]
         s.dir2[:] = [float(t) for t in lp.dir2]
+    s.corner_r = float(lp.corner_r)
+    if lp.axis:
+        s.axis[:] = [float(t) for t in lp.axis]
     keep = None
     if lp.shape == "polygon":
         keep = np.ascontiguousarray(np.asarray(lp.vertices, dtype=np.float64).reshape(-1))
@@ -578,6 +597,15 @@ def native_phys(lp: Loop, da: float = 0.0):
     if _nlib().disco_geo_phys(_ct.byref(s), float(da), _dptr(out)) < 0:
         raise ValueError("invalid loop shape")
     return tuple(out)
+
+
+def native_loop_state(lp: Loop) -> _LoopState:
+    """The loop's state as the reference's constructors leave it (Code/loop.cpp:13-244)."""
+    s, keep = native_shape(lp)
+    st = _LoopState()
+    if _nlib().disco_geo_loop_state(_ct.byref(s), _ct.byref(st)) < 0:
+        raise ValueError("invalid loop shape")
+    return st
 
 
 def native_regular_polygon(sides: int, R: float, m, n):

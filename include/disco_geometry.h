@@ -3,18 +3,19 @@
  * The loop geometry that DISCO's driver builds before every sum -- closed
  * polylines, loopon grids, the mid-plane snap of loop centres -- restated in
  * C++ for batched sample generation (many samples, thousands of loops): the
- * same formulas and operation order as the reference's loop code, so the
- * vertices equal the reference's (tested against the compiled reference's
- * outputs).  Host code only; no device is needed.
+ * same formulas and operation order as the reference's loop code, with the
+ * products fused where the reference's host build fuses them, so the results
+ * are bit-identical to the reference's (tested against the compiled
+ * reference's outputs).  Host code only; no device is needed.
  *
  * Reference interfaces replaced:
  *   disco_geo_regular_polygon  regularPolygon      (Code/loop.cpp:585-608)
  *   disco_geo_snap_mid_plane   snapToMidPlane      (Code/loop.cpp:342-374)
- *   disco_geo_polyline         loop::polyline      (Code/loop.cpp:536-580), sharp
- *                              shapes and circles
+ *   disco_geo_polyline         loop::polyline      (Code/loop.cpp:478-580)
  *   disco_geo_loopons          loop::generateLoopons / insideUV
- *                              (Code/loop.cpp:253-316), sharp shapes
+ *                              (Code/loop.cpp:253-316)
  *   disco_geo_phys             the LoopPhys of a loop (Code/loop.cpp:25-29)
+ *   disco_geo_loop_state       the loop constructors    (Code/loop.cpp:13-244)
  *   disco_geo_polylines        the driver's per-loop LoopData assembly
  *                              (Code/main.cpp:689-711), for many loops at once
  */
@@ -44,7 +45,26 @@ typedef struct {
     double dir1[3], dir2[3];   /* parallelogram edge directions                        */
     int nvert;                 /* polygon                                              */
     const double *vert;        /* polygon: 3 * nvert offsets from the centre           */
+    double corner_r;           /* rect, polygon: corner rounding radius (0 = sharp)    */
+    double axis[3];            /* rect: Side (the half_m axis); polygon: grid axis u;  */
+                               /* all zero = the default axis                          */
 } disco_loop_shape;
+
+/* A loop as the reference's constructors leave it (class loop's state,
+ * Code/loop.hpp:18-45): what the prismatic slab selection reads. */
+#define DISCO_MAX_STATE_VERT 128
+typedef struct {
+    int shape;                 /* DISCO_SHAPE_*                                        */
+    int nvert;                 /* polygon vertices                                     */
+    double center[3];
+    double m[3], n[3];         /* unit Burgers direction and plane normal              */
+    double u[3], v[3];         /* in-plane grid axes (oblique for a parallelogram)     */
+    double bmag;               /* |b|                                                  */
+    double radius, half_m, half_p, r_corner;
+    double pu[DISCO_MAX_STATE_VERT], pv[DISCO_MAX_STATE_VERT];   /* polygon P (u, v)   */
+    double qu[DISCO_MAX_STATE_VERT], qv[DISCO_MAX_STATE_VERT];   /* inner polygon Q    */
+    double nu[DISCO_MAX_STATE_VERT], nv[DISCO_MAX_STATE_VERT];   /* inward side normals */
+} disco_loop_state;
 
 /* Return values: >= 0 the number of points (written when <= cap), < 0 an
  * error: DISCO_GEO_ERR_ARG for an invalid shape or argument. */
@@ -60,8 +80,9 @@ int disco_geo_snap_mid_plane(const double c[3], const double n[3], double a0, in
                              int nz, int lattice, double out[3], double *spacing);
 /* Closed contour, counter-clockwise about n: a circle as nseg chords of the
  * area-preserving radius (nseg <= 0: as many as the sagitta eps needs), a
- * sharp shape by its corners, or (subdivide > 0) each side cut into collinear
- * pieces so the contour has `subdivide` segments. */
+ * sharp shape by its corners and a rounded corner by chords of sagitta <= eps,
+ * or (subdivide > 0, sharp shapes) each side cut into collinear pieces so the
+ * contour has `subdivide` segments. */
 int disco_geo_polyline(const disco_loop_shape *s, double eps, int nseg, int subdivide,
                        double *vx, double *vy, double *vz, int cap);
 /* Loopon grid of spacing da inside the loop. */
@@ -70,6 +91,11 @@ int disco_geo_loopons(const disco_loop_shape *s, double da, double *x, double *y
 /* phys = (|b|, |b|, |b|, m/|m|, n/|n|, dS) with dS = da^2 (times |dir1 x dir2| for a
  * parallelogram), or 0 for da <= 0. */
 int disco_geo_phys(const disco_loop_shape *s, double da, double phys[10]);
+/* The loop's state after construction (initCommon, defaultAxes, the shape's
+ * constructor: Code/loop.cpp:13-244).  Returns 1, or DISCO_GEO_ERR_ARG for an
+ * invalid shape (also a polygon that is not strictly convex or has more than
+ * DISCO_MAX_STATE_VERT vertices). */
+int disco_geo_loop_state(const disco_loop_shape *s, disco_loop_state *out);
 /* Many loops' polylines into concatenated arrays in one call:
  * offsets[l] .. offsets[l + 1] are loop l's vertices; nseg / subdivide per
  * loop (may be NULL: 0).  Returns the total vertex count (arrays written when

@@ -119,6 +119,9 @@ def ref():
         R.ref_snap_mid_plane.argtypes = [_dp, _dp, _d, _i, _i, _i, _i, _dp, _dp]
         R.ref_lattice.argtypes = [_i, _i, _i, _i, _d, _dp, _dp, _dp, _i]
         R.ref_lattice.restype = _i
+        R.ref_prismatic_slab.argtypes = [_i, _dp, _dp, _dp, _dp, _d, _d, _i, _dp, _dp, _dp, _d,
+                                         _d, _d, _ip, _ip, _i, _dp]
+        R.ref_prismatic_slab.restype = _i
         _ref = R
     return _ref
 
@@ -264,12 +267,19 @@ def ref_kroupa_cpu_loopon(x, y, z, loops, box):
     return ux, uy, uz, sec
 
 
-SHAPE_CODE = {"circle": 0, "rect": 1, "parallelogram": 2, "polygon": 3}
+SHAPE_CODE = {"circle": 0, "rect": 1, "parallelogram": 2, "polygon": 3, "polygon_vertices": 4}
+
+
+def _params(params):
+    """Shape parameters, zero-padded (optional trailing directions read as empty)."""
+    p = np.zeros(max(16, len(params) + 8))
+    p[:len(params)] = params
+    return p
 
 
 def ref_polyline(shape, params, c, m, n, a0, eps, cap=4096):
     out = np.zeros(3 * cap)
-    p = _as3(params)
+    p = _params(params)
     nv = ref().ref_loop_polyline(SHAPE_CODE[shape], _ptr(p), _ptr(_as3(c)), _ptr(_as3(m)),
                                  _ptr(_as3(n)), a0, eps, _ptr(out), cap)
     return out[:3 * min(nv, cap)].reshape(-1, 3)
@@ -278,14 +288,14 @@ def ref_polyline(shape, params, c, m, n, a0, eps, cap=4096):
 def ref_loop_phys(shape, params, da, c, m, n, a0):
     phys = np.zeros(10)
     area = np.zeros(1)
-    ref().ref_loop_phys(SHAPE_CODE[shape], _ptr(_as3(params)), da, _ptr(_as3(c)), _ptr(_as3(m)),
+    ref().ref_loop_phys(SHAPE_CODE[shape], _ptr(_params(params)), da, _ptr(_as3(c)), _ptr(_as3(m)),
                         _ptr(_as3(n)), a0, _ptr(phys), _ptr(area))
     return phys, float(area[0])
 
 
 def ref_loopons(shape, params, da, c, m, n, a0, cap=200000):
     out = np.zeros(3 * cap)
-    npn = ref().ref_loop_loopons(SHAPE_CODE[shape], _ptr(_as3(params)), da, _ptr(_as3(c)),
+    npn = ref().ref_loop_loopons(SHAPE_CODE[shape], _ptr(_params(params)), da, _ptr(_as3(c)),
                                  _ptr(_as3(m)), _ptr(_as3(n)), a0, _ptr(out), cap)
     return out[:3 * min(npn, cap)].reshape(-1, 3)
 
@@ -309,6 +319,20 @@ def ref_lattice(lattice, nx, ny, nz, a0):
     x, y, z = np.zeros(cap), np.zeros(cap), np.zeros(cap)
     ref().ref_lattice(1 if lattice == "FCC" else 0, nx, ny, nz, a0, _ptr(x), _ptr(y), _ptr(z), cap)
     return x, y, z
+
+
+def ref_prismatic_slab(shape, params, c, m, n, a0, spacing, x, y, z, box):
+    """The reference's prismaticSlab: (atoms, side, info) with info = (type, planes,
+    height, spacing, b0, b1, b2), type 0 glide, 1 vacancy, 2 interstitial."""
+    x, y, z = _atoms(x, y, z)
+    cap = x.shape[0]
+    atoms, side = np.zeros(cap, dtype=np.int32), np.zeros(cap, dtype=np.int32)
+    info = np.zeros(7)
+    ns = ref().ref_prismatic_slab(SHAPE_CODE[shape], _ptr(_params(params)), _ptr(_as3(c)),
+                                  _ptr(_as3(m)), _ptr(_as3(n)), a0, spacing, cap, _ptr(x),
+                                  _ptr(y), _ptr(z), float(box[0]), float(box[1]), float(box[2]),
+                                  _ptr(atoms, _ip), _ptr(side, _ip), cap, _ptr(info))
+    return atoms[:ns], side[:ns], info
 
 
 # ---------------------------------------------------------------- output stage

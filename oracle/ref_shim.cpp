@@ -14,6 +14,7 @@
 //   ref_loop_*            -> class loop, regularPolygon, snapToMidPlane
 //                            (Code/loop.cpp:64-244,253-316,426-435,536-608,342-374)
 //   ref_lattice           -> generateLattice       (Code/lattice.hpp:35-51)
+//   ref_prismatic_slab    -> prismaticSlab         (Code/prismatic.cpp:30-74)
 #include <cstring>
 #include <iostream>
 #include <sstream>
@@ -22,6 +23,7 @@
 #include "backend.hpp"
 #include "barnett.hpp"
 #include "loop.hpp"
+#include "prismatic.hpp"
 
 using std::vector;
 
@@ -38,14 +40,23 @@ template <typename F> void quiet(F fn)
 }
 
 // Shape codes for ref_loop_make
-enum { SHAPE_CIRCLE = 0, SHAPE_RECT = 1, SHAPE_PARALLELOGRAM = 2, SHAPE_POLYGON = 3 };
+enum { SHAPE_CIRCLE = 0, SHAPE_RECT = 1, SHAPE_PARALLELOGRAM = 2, SHAPE_POLYGON = 3,
+       SHAPE_POLYGON_VERTICES = 4 };
+
+// an optional direction: all zero = empty (the default axis)
+vector<double> opt3(const double* p)
+{
+    if (p[0] == 0. && p[1] == 0. && p[2] == 0.) return {};
+    return v3(p);
+}
 
 // Builds a loop; params by shape:
 //   circle:        p[0] = radius
-//   rect:          p[0] = half_m, p[1] = half_p, p[2] = corner_r
+//   rect:          p[0] = half_m, p[1] = half_p, p[2] = corner_r, p[3..5] = side
 //   parallelogram: p[0] = half1, p[1] = half2, p[2..4] = dir1, p[5..7] = dir2
-//   polygon:       p[0] = sides, p[1] = circumradius R, p[2] = corner_r
+//   polygon:       p[0] = sides, p[1] = circumradius R, p[2] = corner_r, p[3..5] = axis
 //                  (vertices from regularPolygon(sides, R, {}, m, n))
+//   polygon by vertices: p[0] = count, p[1] = corner_r, p[2..4] = axis, p[5..] = vertices
 loop make_loop(int shape, const double* p, double da, const double* c, const double* m,
                const double* n, double a0)
 {
@@ -54,12 +65,17 @@ loop make_loop(int shape, const double* p, double da, const double* c, const dou
         case SHAPE_CIRCLE:
             return loop(da, p[0], v3(c), v3(m), v3(n), a0, one);
         case SHAPE_RECT:
-            return loop(da, p[0], p[1], p[2], v3(c), v3(m), v3(n), a0, one);
+            return loop(da, p[0], p[1], p[2], v3(c), v3(m), v3(n), a0, one, opt3(p + 3));
         case SHAPE_PARALLELOGRAM:
             return loop(da, p[0], p[1], v3(p + 2), v3(p + 5), v3(c), v3(m), v3(n), a0, one);
-        default:
+        case SHAPE_POLYGON:
             return loop(da, regularPolygon((int)p[0], p[1], {}, v3(m), v3(n)), p[2], v3(c),
-                        v3(m), v3(n), a0, one);
+                        v3(m), v3(n), a0, one, opt3(p + 3));
+        default: {
+            vector<vector<double>> V;
+            for (int k = 0; k < (int)p[0]; k++) V.push_back(v3(p + 5 + 3 * k));
+            return loop(da, V, p[1], v3(c), v3(m), v3(n), a0, one, opt3(p + 2));
+        }
     }
 }
 
@@ -210,6 +226,27 @@ int ref_lattice(int lattice, int nx, int ny, int nz, double a0, double* x, doubl
         generateLattice(lattice ? Lattice::FCC : Lattice::BCC, nx, ny, nz, a0);
     for (int i = 0; i < (int)A.size() && i < cap; i++) { x[i] = A[i][0]; y[i] = A[i][1]; z[i] = A[i][2]; }
     return (int)A.size();
+}
+
+// prismaticSlab (Code/prismatic.cpp:30-74) of one loop over SoA atoms; returns
+// the slab size (atoms / side written up to cap) and info = (type, planes,
+// height, spacing, b0, b1, b2), type 0 none, 1 vacancy, 2 interstitial.
+int ref_prismatic_slab(int shape, const double* p, const double* c, const double* m,
+                       const double* n, double a0, double spacing, int natoms, const double* x,
+                       const double* y, const double* z, double lx, double ly, double lz,
+                       int* atoms, int* side, int cap, double* info)
+{
+    const loop lp = make_loop(shape, p, 1.0, c, m, n, a0);
+    const vector<double> X(x, x + natoms), Y(y, y + natoms), Z(z, z + natoms);
+    const SlabResult r = prismaticSlab(lp, spacing, X, Y, Z, lx, ly, lz);
+    const int ns = (int)r.atoms.size();
+    for (int k = 0; k < ns && k < cap; k++) { atoms[k] = r.atoms[k]; side[k] = r.side[k]; }
+    info[0] = r.type == MatterType::NONE ? 0 : (r.type == MatterType::VACANCY ? 1 : 2);
+    info[1] = r.planes;
+    info[2] = r.height;
+    info[3] = r.spacing;
+    for (int i = 0; i < 3; i++) info[4 + i] = r.b[i];
+    return ns;
 }
 
 }  // extern "C"
