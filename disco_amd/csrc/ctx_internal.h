@@ -23,6 +23,9 @@ struct disco_ctx {
     char* d_text = nullptr;
     size_t cap_text = 0;
     char* h_stage[2] = {nullptr, nullptr};
+    // prismatic slab selection (grow-only): loop copy, block counts / offsets, flags
+    void* d_slab_scratch = nullptr;
+    size_t cap_slab_scratch = 0;
     // optional timing of the text kernels of the last format call (count, scan, write)
     int text_profile = 0;
     hipEvent_t text_ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -37,8 +40,10 @@ inline int fail(disco_ctx* ctx, int code, const std::string& msg)
     return code;
 }
 
-// Frees the output stage's buffers (writers.hip); called by disco_hip_close.
+// Free the output stage's (writers.hip) and the slab selection's
+// (prismatic.hip) buffers; called by disco_hip_close.
 void release_output_buffers(disco_ctx* ctx);
+void release_prismatic_buffers(disco_ctx* ctx);
 
 inline double now_s()
 {

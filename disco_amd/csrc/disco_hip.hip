@@ -1076,6 +1076,7 @@ int disco_hip_close(disco_ctx* ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->d_atoms) (void)hipFree(ctx->d_atoms);
     disco::release_output_buffers(ctx);
+    disco::release_prismatic_buffers(ctx);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

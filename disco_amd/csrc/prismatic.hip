@@ -1,0 +1,317 @@
+// prismatic.hip -- prismatic slab selection on gfx950 (include/disco_prismatic.h).
+//
+// Reference behaviour replaced: prismaticSlab (Code/prismatic.cpp:30-74), an
+// O(atoms) host scan per loop with minImage, the signed distance to the cut
+// and loop::contains / insideUV (Code/loop.cpp:281-330).
+//
+// Three launches per loop, no waits between workgroups:
+//   slab_flag_kernel     one atom per lane: the predicate, a flag byte per atom
+//                        (0 / +1 / -1) and the count of selected atoms per
+//                        256-atom block (wave ballots);
+//   slab_scan_kernel     exclusive scan of the block counts (one workgroup);
+//   slab_scatter_kernel  each selected atom writes its index and side at its
+//                        block offset + its rank in the block: ascending
+//                        order, as the reference's push_back.
+// HBM traffic per atom: 24 B of positions + 1 B flag written, then 1 B read.
+//
+// Rounding: the predicate is the reference's expression by expression; the
+// products its host build fuses into FMAs (GCC, x86-64 with FMA) are fused
+// here with fma() in the same places and the rest is not contracted, and
+// double division / sqrt are correctly rounded on this device, so every
+// comparison sees the reference's values bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "ctx_internal.h"
+#include "disco_prismatic.h"
+
+#pragma clang fp contract(off)
+
+using namespace disco;
+
+namespace {
+
+constexpr int kSlabBlock = 256;
+constexpr int kScanThreads = 1024;
+constexpr int kScanPer = 16;   // block counts per scan thread per round
+
+// The loop as the predicate reads it (device copy).
+struct SlabLoop {
+    disco_loop_state s;
+    double cuv;       // u . v
+    double det;       // 1 - (u . v)^2
+    double t_lo, t_hi;
+    double spacing;
+    double lx, ly, lz;
+};
+
+__device__ __forceinline__ double dmax(double a, double b) { return a < b ? b : a; }
+__device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
+
+// r - L floor(r / L + 1/2)  (Code/prismatic.cpp:28-31), the product fused
+__device__ __forceinline__ double min_image(double r, double L)
+{
+    return fma(-L, floor(r / L + 0.5), r);
+}
+
+// Code/loop.cpp:281-316
+__device__ bool inside_uv(const disco_loop_state& L, double u, double v)
+{
+    const double eps = 1e-9;
+    if (L.shape == DISCO_SHAPE_CIRCLE) return sqrt(fma(u, u, v * v)) < L.radius - eps;
+    if (L.shape == DISCO_SHAPE_POLYGON) {
+        const int N = L.nvert;
+        bool in_q = true;
+        for (int i = 0; i < N && in_q; i++)
+            if (fma(L.nu[i], u - L.pu[i], L.nv[i] * (v - L.pv[i])) < L.r_corner - eps) in_q = false;
+        if (in_q) return true;
+        if (L.r_corner <= 0.0) return false;
+        double d2 = 1e300;
+        for (int i = 0; i < N; i++) {
+            const int j = i + 1 == N ? 0 : i + 1;
+            const double eu = L.qu[j] - L.qu[i], ev = L.qv[j] - L.qv[i];
+            const double l2 = fma(eu, eu, ev * ev);
+            double t = l2 > 0.0 ? fma(u - L.qu[i], eu, (v - L.qv[i]) * ev) / l2 : 0.0;
+            t = dmin(dmax(t, 0.0), 1.0);
+            const double wu = fma(-t, eu, u - L.qu[i]);
+            const double wv = fma(-t, ev, v - L.qv[i]);
+            d2 = dmin(d2, fma(wu, wu, wv * wv));
+        }
+        return d2 <= fma(L.r_corner, L.r_corner, eps);
+    }
+    const double du = dmax(fabs(u) - dmax(L.half_m - L.r_corner, 0.0), 0.0);
+    const double dv = dmax(fabs(v) - dmax(L.half_p - L.r_corner, 0.0), 0.0);
+    return fma(du, du, dv * dv) <= fma(L.r_corner, L.r_corner, eps);
+}
+
+// +1 / -1 for a slab atom (side of the cut), 0 otherwise
+__device__ __forceinline__ int slab_side(const SlabLoop& P, double x, double y, double z)
+{
+    const disco_loop_state& L = P.s;
+    const double r0 = min_image(x - L.center[0], P.lx);
+    const double r1 = min_image(y - L.center[1], P.ly);
+    const double r2 = min_image(z - L.center[2], P.lz);
+    const double s = fma(r2, L.n[2], fma(r0, L.n[0], r1 * L.n[1]));
+    const double t = s / P.spacing;
+    if (t < P.t_lo || t >= P.t_hi) return 0;
+    // loop::contains: oblique coordinates of rel on (u, v)
+    const double ru = fma(r2, L.u[2], fma(r0, L.u[0], r1 * L.u[1]));
+    const double rv = fma(r2, L.v[2], fma(r0, L.v[0], r1 * L.v[1]));
+    if (!inside_uv(L, fma(-P.cuv, rv, ru) / P.det, fma(-P.cuv, ru, rv) / P.det)) return 0;
+    return s > 0.0 ? 1 : -1;
+}
+
+__global__ __launch_bounds__(kSlabBlock) void slab_flag_kernel(
+    const SlabLoop* __restrict__ loop, long long n, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, signed char* __restrict__ flags,
+    unsigned* __restrict__ block_counts)
+{
+    __shared__ unsigned wcount[kSlabBlock / 64];
+    const long long k = (long long)blockIdx.x * kSlabBlock + threadIdx.x;
+    int f = 0;
+    if (k < n) {
+        f = slab_side(*loop, x[k], y[k], z[k]);
+        flags[k] = (signed char)f;
+    }
+    const unsigned long long ball = __ballot(f != 0);
+    if ((threadIdx.x & 63) == 0) wcount[threadIdx.x >> 6] = (unsigned)__popcll(ball);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = 0;
+        for (int w = 0; w < kSlabBlock / 64; w++) t += wcount[w];
+        block_counts[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of nb counts in one workgroup: each thread sums kScanPer
+// consecutive counts per round, the thread sums are scanned across the
+// workgroup, then each thread writes its offsets.
+__global__ __launch_bounds__(kScanThreads) void slab_scan_kernel(
+    const unsigned* __restrict__ counts, unsigned long long* __restrict__ offsets, long long nb,
+    unsigned long long* __restrict__ total)
+{
+    __shared__ unsigned long long wsum[kScanThreads / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    unsigned long long carry = 0;
+    for (long long base = 0; base < nb; base += (long long)kScanThreads * kScanPer) {
+        const long long i0 = base + (long long)t * kScanPer;
+        unsigned long long mine = 0;
+        for (int j = 0; j < kScanPer; j++)
+            if (i0 + j < nb) mine += counts[i0 + j];
+        // inclusive scan of the thread sums within the wave
+        unsigned long long inc = mine;
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned long long v = __shfl_up(inc, o);
+            if (lane >= o) inc += v;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        unsigned long long before = 0, round = 0;
+        for (int w = 0; w < kScanThreads / 64; w++) {
+            before += w < wave ? wsum[w] : 0ull;
+            round += wsum[w];
+        }
+        unsigned long long run = carry + before + inc - mine;
+        for (int j = 0; j < kScanPer; j++)
+            if (i0 + j < nb) {
+                offsets[i0 + j] = run;
+                run += counts[i0 + j];
+            }
+        carry += round;
+        __syncthreads();   // wsum is rewritten next round
+    }
+    if (t == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kSlabBlock) void slab_scatter_kernel(
+    long long n, const signed char* __restrict__ flags,
+    const unsigned long long* __restrict__ offsets, int* __restrict__ atoms,
+    int* __restrict__ side, long long cap)
+{
+    __shared__ unsigned wcount[kSlabBlock / 64];
+    const long long k = (long long)blockIdx.x * kSlabBlock + threadIdx.x;
+    const int f = k < n ? flags[k] : 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long ball = __ballot(f != 0);
+    if (lane == 0) wcount[wave] = (unsigned)__popcll(ball);
+    __syncthreads();
+    if (!f) return;
+    unsigned before = 0;
+    for (int w = 0; w < wave; w++) before += wcount[w];
+    const unsigned rank = before + (unsigned)__popcll(ball & ((1ull << lane) - 1ull));
+    const unsigned long long pos = offsets[blockIdx.x] + rank;
+    if ((long long)pos < cap) {
+        atoms[pos] = (int)k;
+        side[pos] = f;
+    }
+}
+
+int ensure(disco_ctx* ctx, void** p, size_t* cap, size_t bytes)
+{
+    if (*cap >= bytes) return DISCO_OK;
+    if (*p) DISCO_HIP_TRY(ctx, hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+    DISCO_HIP_TRY(ctx, hipMalloc(p, bytes));
+    *cap = bytes;
+    return DISCO_OK;
+}
+
+double dot3(const double* a, const double* b)
+{
+    return std::fma(a[2], b[2], std::fma(a[0], b[0], a[1] * b[1]));
+}
+
+}  // namespace
+
+namespace disco {
+
+void release_prismatic_buffers(disco_ctx* ctx)
+{
+    if (ctx->d_slab_scratch) (void)hipFree(ctx->d_slab_scratch);
+    ctx->d_slab_scratch = nullptr;
+    ctx->cap_slab_scratch = 0;
+}
+
+}  // namespace disco
+
+extern "C" {
+
+int disco_hip_matter_type(const disco_loop_state* loop)
+{
+    if (!loop) return -1;
+    const double mn = dot3(loop->m, loop->n);
+    if (std::fabs(mn) < 1e-6) return DISCO_MATTER_NONE;
+    return mn > 0.0 ? DISCO_MATTER_VACANCY : DISCO_MATTER_INTERSTITIAL;
+}
+
+int disco_hip_prismatic_slab(disco_ctx* ctx, const disco_loop_state* loop, double spacing,
+                             long long n, const double* d_x, const double* d_y,
+                             const double* d_z, double lx, double ly, double lz, int* d_atoms,
+                             int* d_side, long long cap, disco_slab_info* info, void* stream)
+{
+    if (!ctx) return DISCO_ERR_ARG;
+    if (!loop || !info) return fail(ctx, DISCO_ERR_ARG, "null loop or info");
+    std::memset(info, 0, sizeof(*info));
+    if (loop->shape < DISCO_SHAPE_CIRCLE || loop->shape > DISCO_SHAPE_PARALLELOGRAM)
+        return fail(ctx, DISCO_ERR_ARG, "unknown loop shape");
+    if (loop->shape == DISCO_SHAPE_POLYGON &&
+        (loop->nvert < 3 || loop->nvert > DISCO_MAX_STATE_VERT))
+        return fail(ctx, DISCO_ERR_ARG, "polygon vertex count out of range");
+    if (!(spacing > 0.0) || !std::isfinite(spacing))
+        return fail(ctx, DISCO_ERR_ARG, "plane spacing must be positive");
+    if (!(lx > 0.0) || !(ly > 0.0) || !(lz > 0.0))
+        return fail(ctx, DISCO_ERR_ARG, "box lengths must be positive");
+    if (n < 0 || cap < 0) return fail(ctx, DISCO_ERR_ARG, "negative size");
+    if (n > 0x7fffffffll) return fail(ctx, DISCO_ERR_TOO_LARGE, "atom indices are int");
+
+    // matterType, then the slab height in planes (Code/prismatic.cpp:37-53)
+    info->type = disco_hip_matter_type(loop);
+    if (info->type == DISCO_MATTER_NONE) return DISCO_OK;
+    for (int i = 0; i < 3; i++) info->b[i] = loop->bmag * loop->m[i];
+    info->height = std::fabs(dot3(info->b, loop->n));
+    info->spacing = spacing;
+    const double ratio = info->height / spacing;
+    info->planes = (int)std::lround(ratio);
+    if (info->planes < 1 || std::fabs(ratio - info->planes) > 1e-6)
+        return fail(ctx, DISCO_ERR_ARG,
+                    "|b.n| is not a whole number of atomic-plane spacings: this b cannot close "
+                    "on this habit plane");
+    if (n == 0) return DISCO_OK;
+    if (!d_x || !d_y || !d_z) return fail(ctx, DISCO_ERR_ARG, "null device array");
+    if (cap > 0 && (!d_atoms || !d_side)) return fail(ctx, DISCO_ERR_ARG, "null output array");
+
+    SlabLoop P;
+    std::memset(&P, 0, sizeof(P));
+    P.s = *loop;
+    P.cuv = dot3(loop->u, loop->v);
+    P.det = std::fma(-P.cuv, P.cuv, 1.0);
+    P.t_lo = -std::floor(info->planes / 2.);
+    P.t_hi = std::ceil(info->planes / 2.);
+    P.spacing = spacing;
+    P.lx = lx;
+    P.ly = ly;
+    P.lz = lz;
+
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const long long nb = (n + kSlabBlock - 1) / kSlabBlock;
+    // scratch: loop | total | block counts | block offsets | flags
+    const size_t o_total = (sizeof(SlabLoop) + 255) & ~(size_t)255;
+    const size_t o_counts = o_total + 256;
+    const size_t o_offs = (o_counts + (size_t)nb * 4 + 255) & ~(size_t)255;
+    const size_t o_flags = (o_offs + (size_t)nb * 8 + 255) & ~(size_t)255;
+    const size_t need = o_flags + (size_t)nb * kSlabBlock;
+    int rc = ensure(ctx, &ctx->d_slab_scratch, &ctx->cap_slab_scratch, need);
+    if (rc) return rc;
+    char* sc = static_cast<char*>(ctx->d_slab_scratch);
+    SlabLoop* d_loop = reinterpret_cast<SlabLoop*>(sc);
+    unsigned long long* d_total = reinterpret_cast<unsigned long long*>(sc + o_total);
+    unsigned* d_counts = reinterpret_cast<unsigned*>(sc + o_counts);
+    unsigned long long* d_offs = reinterpret_cast<unsigned long long*>(sc + o_offs);
+    signed char* d_flags = reinterpret_cast<signed char*>(sc + o_flags);
+    hipStream_t s = (hipStream_t)stream;
+    DISCO_HIP_TRY(ctx, hipMemcpyAsync(d_loop, &P, sizeof(P), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(slab_flag_kernel, dim3((unsigned)nb), dim3(kSlabBlock), 0, s, d_loop, n,
+                       d_x, d_y, d_z, d_flags, d_counts);
+    DISCO_HIP_TRY(ctx, hipGetLastError());
+    hipLaunchKernelGGL(slab_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, d_counts, d_offs, nb,
+                       d_total);
+    DISCO_HIP_TRY(ctx, hipGetLastError());
+    if (cap > 0) {
+        hipLaunchKernelGGL(slab_scatter_kernel, dim3((unsigned)nb), dim3(kSlabBlock), 0, s, n,
+                           d_flags, d_offs, d_atoms, d_side, cap);
+        DISCO_HIP_TRY(ctx, hipGetLastError());
+    }
+    unsigned long long total = 0;
+    DISCO_HIP_TRY(ctx, hipMemcpyAsync(&total, d_total, sizeof(total), hipMemcpyDeviceToHost, s));
+    DISCO_HIP_TRY(ctx, hipStreamSynchronize(s));
+    info->count = (long long)total;
+    if (info->count > cap)
+        return fail(ctx, DISCO_ERR_TOO_LARGE, "slab larger than the output capacity");
+    return DISCO_OK;
+}
+
+}  // extern "C"

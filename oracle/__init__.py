@@ -30,6 +30,9 @@ loopon_sum (+-img images)     golden kroupaCPU output at +-1 (C3 loop, 4,096
                               atoms); +-N = sum of single-image fields
 line_sum_ref32                golden barnettLoop<float> vectors (the float
                               instantiation the reference's --fp32 uses)
+prismatic_slab, matter_type   the compiled reference's prismaticSlab on every
+                              loop shape (rounded corners, axes, polygons);
+                              golden slabs; brute force in exact arithmetic
 final_positions               exact rational arithmetic of the reference's
                               expression shapes under GCC contraction; wrap
                               closed forms; identity U leaves wrapped + L/2
@@ -87,6 +90,11 @@ def lib():
                                         _dp, _d, _d, _d, _i]
         L.oracle_line_sum_ref32.argtypes = [_i, _dp, _dp, _dp, _dp, _dp, _dp, _i, _ip, _dp, _dp,
                                             _dp, _dp, _dp, _d, _d, _d, _i, _fp]
+        L.oracle_matter_type.argtypes = [ctypes.c_void_p]
+        L.oracle_matter_type.restype = _i
+        L.oracle_prismatic_slab.argtypes = [ctypes.c_void_p, _d, ctypes.c_longlong, _dp, _dp,
+                                            _dp, _d, _d, _d, _ip, _ip, ctypes.c_longlong, _dp]
+        L.oracle_prismatic_slab.restype = ctypes.c_longlong
         _lib = L
     return _lib
 
@@ -319,6 +327,29 @@ def ref_lattice(lattice, nx, ny, nz, a0):
     x, y, z = np.zeros(cap), np.zeros(cap), np.zeros(cap)
     ref().ref_lattice(1 if lattice == "FCC" else 0, nx, ny, nz, a0, _ptr(x), _ptr(y), _ptr(z), cap)
     return x, y, z
+
+
+def prismatic_slab(state, spacing, x, y, z, box):
+    """prismaticSlab restated (Code/prismatic.cpp:30-74) for a constructed loop
+    state (a disco_loop_state ctypes structure): (atoms, side, info), info =
+    (type, planes, height, spacing, b0, b1, b2); raises ValueError where the
+    reference aborts (|b.n| not a whole number of plane spacings)."""
+    x, y, z = _atoms(x, y, z)
+    cap = x.shape[0]
+    atoms, side = np.zeros(cap, dtype=np.int32), np.zeros(cap, dtype=np.int32)
+    info = np.zeros(7)
+    ns = lib().oracle_prismatic_slab(ctypes.addressof(state), float(spacing), cap, _ptr(x),
+                                     _ptr(y), _ptr(z), float(box[0]), float(box[1]),
+                                     float(box[2]), _ptr(atoms, _ip), _ptr(side, _ip), cap,
+                                     _ptr(info))
+    if ns < 0:
+        raise ValueError("|b.n| is not a whole number of atomic-plane spacings")
+    return atoms[:ns], side[:ns], info
+
+
+def matter_type(state) -> int:
+    """matterType (Code/prismatic.cpp:12-18): 0 glide, 1 vacancy, 2 interstitial."""
+    return lib().oracle_matter_type(ctypes.addressof(state))
 
 
 def ref_prismatic_slab(shape, params, c, m, n, a0, spacing, x, y, z, box):
