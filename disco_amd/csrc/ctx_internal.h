@@ -26,6 +26,7 @@ struct disco_ctx {
     // prismatic slab selection (grow-only): loop copy, block counts / offsets, flags
     void* d_slab_scratch = nullptr;
     size_t cap_slab_scratch = 0;
+    void* h_slab_stage = nullptr;   // pinned: the loop copy and the slab total
     // optional timing of the text kernels of the last format call (count, scan, write)
     int text_profile = 0;
     hipEvent_t text_ev[4] = {nullptr, nullptr, nullptr, nullptr};

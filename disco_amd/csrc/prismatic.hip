@@ -5,13 +5,14 @@
 // and loop::contains / insideUV (Code/loop.cpp:281-330).
 //
 // Three launches per loop, no waits between workgroups:
-//   slab_flag_kernel     one atom per lane: the predicate, a flag byte per atom
-//                        (0 / +1 / -1) and the count of selected atoms per
-//                        256-atom block (wave ballots);
-//   slab_scan_kernel     exclusive scan of the block counts (one workgroup);
+//   slab_flag_kernel     a workgroup per tile of 4096 atoms, one atom per lane
+//                        per step: the predicate, a flag byte per atom
+//                        (0 / +1 / -1) and the tile's count (wave ballots);
+//   slab_scan_kernel     exclusive scan of the tile counts (one workgroup,
+//                        ~1000 counts for 4M atoms);
 //   slab_scatter_kernel  each selected atom writes its index and side at its
-//                        block offset + its rank in the block: ascending
-//                        order, as the reference's push_back.
+//                        tile offset + its rank in the tile: ascending order,
+//                        as the reference's push_back.
 // HBM traffic per atom: 24 B of positions + 1 B flag written, then 1 B read.
 //
 // Rounding: the predicate is the reference's expression by expression; the
@@ -35,8 +36,8 @@ using namespace disco;
 namespace {
 
 constexpr int kSlabBlock = 256;
+constexpr int kSlabTile = 4096;   // atoms per workgroup (16 per lane)
 constexpr int kScanThreads = 1024;
-constexpr int kScanPer = 16;   // block counts per scan thread per round
 
 // The loop as the predicate reads it (device copy).
 struct SlabLoop {
@@ -104,31 +105,55 @@ __device__ __forceinline__ int slab_side(const SlabLoop& P, double x, double y, 
     return s > 0.0 ? 1 : -1;
 }
 
+// Block-wide exclusive rank of this lane's flag among the workgroup's set
+// flags (wave ballots, wave counts in LDS); returns the workgroup's total.
+__device__ __forceinline__ unsigned block_rank(bool set, unsigned* wcount, unsigned* rank)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long ball = __ballot(set);
+    if (lane == 0) wcount[wave] = (unsigned)__popcll(ball);
+    __syncthreads();
+    unsigned before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kSlabBlock / 64; w++) {
+        before += w < wave ? wcount[w] : 0u;
+        total += wcount[w];
+    }
+    *rank = before + (unsigned)__popcll(ball & ((1ull << lane) - 1ull));
+    __syncthreads();   // wcount is rewritten by the next call
+    return total;
+}
+
+// One workgroup per tile of kSlabTile atoms (kSlabBlock lanes, kSlabTile /
+// kSlabBlock atoms each): the flag of every atom, and the tile's slab count.
 __global__ __launch_bounds__(kSlabBlock) void slab_flag_kernel(
     const SlabLoop* __restrict__ loop, long long n, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z, signed char* __restrict__ flags,
-    unsigned* __restrict__ block_counts)
+    unsigned* __restrict__ tile_counts)
 {
     __shared__ unsigned wcount[kSlabBlock / 64];
-    const long long k = (long long)blockIdx.x * kSlabBlock + threadIdx.x;
-    int f = 0;
-    if (k < n) {
-        f = slab_side(*loop, x[k], y[k], z[k]);
-        flags[k] = (signed char)f;
+    const long long t0 = (long long)blockIdx.x * kSlabTile;
+    unsigned count = 0;
+    for (int j = 0; j < kSlabTile; j += kSlabBlock) {
+        const long long k = t0 + j + threadIdx.x;
+        int f = 0;
+        if (k < n) {
+            f = slab_side(*loop, x[k], y[k], z[k]);
+            flags[k] = (signed char)f;
+        }
+        count += (unsigned)__popcll(__ballot(f != 0));   // wave-uniform
     }
-    const unsigned long long ball = __ballot(f != 0);
-    if ((threadIdx.x & 63) == 0) wcount[threadIdx.x >> 6] = (unsigned)__popcll(ball);
+    if ((threadIdx.x & 63) == 0) wcount[threadIdx.x >> 6] = count;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned t = 0;
         for (int w = 0; w < kSlabBlock / 64; w++) t += wcount[w];
-        block_counts[blockIdx.x] = t;
+        tile_counts[blockIdx.x] = t;
     }
 }
 
-// Exclusive scan of nb counts in one workgroup: each thread sums kScanPer
-// consecutive counts per round, the thread sums are scanned across the
-// workgroup, then each thread writes its offsets.
+// Exclusive scan of the tile counts, one workgroup, one count per lane per
+// round (coalesced), a running carry between rounds.
 __global__ __launch_bounds__(kScanThreads) void slab_scan_kernel(
     const unsigned* __restrict__ counts, unsigned long long* __restrict__ offsets, long long nb,
     unsigned long long* __restrict__ total)
@@ -136,13 +161,11 @@ __global__ __launch_bounds__(kScanThreads) void slab_scan_kernel(
     __shared__ unsigned long long wsum[kScanThreads / 64];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     unsigned long long carry = 0;
-    for (long long base = 0; base < nb; base += (long long)kScanThreads * kScanPer) {
-        const long long i0 = base + (long long)t * kScanPer;
-        unsigned long long mine = 0;
-        for (int j = 0; j < kScanPer; j++)
-            if (i0 + j < nb) mine += counts[i0 + j];
-        // inclusive scan of the thread sums within the wave
+    for (long long base = 0; base < nb; base += kScanThreads) {
+        const long long i = base + t;
+        const unsigned long long mine = i < nb ? counts[i] : 0ull;
         unsigned long long inc = mine;
+#pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const unsigned long long v = __shfl_up(inc, o);
             if (lane >= o) inc += v;
@@ -150,42 +173,41 @@ __global__ __launch_bounds__(kScanThreads) void slab_scan_kernel(
         if (lane == 63) wsum[wave] = inc;
         __syncthreads();
         unsigned long long before = 0, round = 0;
+#pragma unroll
         for (int w = 0; w < kScanThreads / 64; w++) {
             before += w < wave ? wsum[w] : 0ull;
             round += wsum[w];
         }
-        unsigned long long run = carry + before + inc - mine;
-        for (int j = 0; j < kScanPer; j++)
-            if (i0 + j < nb) {
-                offsets[i0 + j] = run;
-                run += counts[i0 + j];
-            }
+        if (i < nb) offsets[i] = carry + before + inc - mine;
         carry += round;
         __syncthreads();   // wsum is rewritten next round
     }
     if (t == 0) *total = carry;
 }
 
+// Each selected atom writes its index and side at its tile's offset plus its
+// rank in the tile: ascending order, as the reference's push_back.
 __global__ __launch_bounds__(kSlabBlock) void slab_scatter_kernel(
     long long n, const signed char* __restrict__ flags,
     const unsigned long long* __restrict__ offsets, int* __restrict__ atoms,
     int* __restrict__ side, long long cap)
 {
     __shared__ unsigned wcount[kSlabBlock / 64];
-    const long long k = (long long)blockIdx.x * kSlabBlock + threadIdx.x;
-    const int f = k < n ? flags[k] : 0;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned long long ball = __ballot(f != 0);
-    if (lane == 0) wcount[wave] = (unsigned)__popcll(ball);
-    __syncthreads();
-    if (!f) return;
-    unsigned before = 0;
-    for (int w = 0; w < wave; w++) before += wcount[w];
-    const unsigned rank = before + (unsigned)__popcll(ball & ((1ull << lane) - 1ull));
-    const unsigned long long pos = offsets[blockIdx.x] + rank;
-    if ((long long)pos < cap) {
-        atoms[pos] = (int)k;
-        side[pos] = f;
+    const long long t0 = (long long)blockIdx.x * kSlabTile;
+    unsigned long long pos0 = offsets[blockIdx.x];
+    for (int j = 0; j < kSlabTile; j += kSlabBlock) {
+        const long long k = t0 + j + threadIdx.x;
+        const int f = k < n ? flags[k] : 0;
+        unsigned rank;
+        const unsigned step = block_rank(f != 0, wcount, &rank);
+        if (f) {
+            const unsigned long long pos = pos0 + rank;
+            if ((long long)pos < cap) {
+                atoms[pos] = (int)k;
+                side[pos] = f;
+            }
+        }
+        pos0 += step;
     }
 }
 
@@ -212,7 +234,9 @@ namespace disco {
 void release_prismatic_buffers(disco_ctx* ctx)
 {
     if (ctx->d_slab_scratch) (void)hipFree(ctx->d_slab_scratch);
+    if (ctx->h_slab_stage) (void)hipHostFree(ctx->h_slab_stage);
     ctx->d_slab_scratch = nullptr;
+    ctx->h_slab_stage = nullptr;
     ctx->cap_slab_scratch = 0;
 }
 
@@ -264,7 +288,15 @@ int disco_hip_prismatic_slab(disco_ctx* ctx, const disco_loop_state* loop, doubl
     if (!d_x || !d_y || !d_z) return fail(ctx, DISCO_ERR_ARG, "null device array");
     if (cap > 0 && (!d_atoms || !d_side)) return fail(ctx, DISCO_ERR_ARG, "null output array");
 
-    SlabLoop P;
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    // pinned staging: the loop copy (H2D) and the slab total (D2H)
+    if (!ctx->h_slab_stage)
+        DISCO_HIP_TRY(ctx, hipHostMalloc(&ctx->h_slab_stage, sizeof(SlabLoop) + 256,
+                                         hipHostMallocDefault));
+    SlabLoop& P = *static_cast<SlabLoop*>(ctx->h_slab_stage);
+    unsigned long long* h_total =
+        reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->h_slab_stage) +
+                                              ((sizeof(SlabLoop) + 63) & ~(size_t)63));
     std::memset(&P, 0, sizeof(P));
     P.s = *loop;
     P.cuv = dot3(loop->u, loop->v);
@@ -276,14 +308,13 @@ int disco_hip_prismatic_slab(disco_ctx* ctx, const disco_loop_state* loop, doubl
     P.ly = ly;
     P.lz = lz;
 
-    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const long long nb = (n + kSlabBlock - 1) / kSlabBlock;
-    // scratch: loop | total | block counts | block offsets | flags
+    const long long nb = (n + kSlabTile - 1) / kSlabTile;
+    // scratch: loop | total | tile counts | tile offsets | flags
     const size_t o_total = (sizeof(SlabLoop) + 255) & ~(size_t)255;
     const size_t o_counts = o_total + 256;
     const size_t o_offs = (o_counts + (size_t)nb * 4 + 255) & ~(size_t)255;
     const size_t o_flags = (o_offs + (size_t)nb * 8 + 255) & ~(size_t)255;
-    const size_t need = o_flags + (size_t)nb * kSlabBlock;
+    const size_t need = o_flags + (size_t)nb * kSlabTile;
     int rc = ensure(ctx, &ctx->d_slab_scratch, &ctx->cap_slab_scratch, need);
     if (rc) return rc;
     char* sc = static_cast<char*>(ctx->d_slab_scratch);
@@ -305,10 +336,9 @@ int disco_hip_prismatic_slab(disco_ctx* ctx, const disco_loop_state* loop, doubl
                            d_flags, d_offs, d_atoms, d_side, cap);
         DISCO_HIP_TRY(ctx, hipGetLastError());
     }
-    unsigned long long total = 0;
-    DISCO_HIP_TRY(ctx, hipMemcpyAsync(&total, d_total, sizeof(total), hipMemcpyDeviceToHost, s));
+    DISCO_HIP_TRY(ctx, hipMemcpyAsync(h_total, d_total, sizeof(*h_total), hipMemcpyDeviceToHost, s));
     DISCO_HIP_TRY(ctx, hipStreamSynchronize(s));
-    info->count = (long long)total;
+    info->count = (long long)*h_total;
     if (info->count > cap)
         return fail(ctx, DISCO_ERR_TOO_LARGE, "slab larger than the output capacity");
     return DISCO_OK;
