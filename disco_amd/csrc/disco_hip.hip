@@ -67,6 +67,7 @@ struct DevMath {
     DISCO_HD static double log1p_d(double x) { return log1p(x); }
     DISCO_HD static double atan2_d(double y, double x) { return atan2(y, x); }
     DISCO_HD static double fabs_d(double x) { return fabs(x); }
+    DISCO_HD static bool signbit_d(double x) { return signbit(x); }
     DISCO_HD static double fmax_d(double a, double b) { return fmax(a, b); }
     DISCO_HD static int exponent_d(double x) { return __builtin_amdgcn_frexp_exp(x); }
     DISCO_HD static double ldexp_d(double x, int e) { return __builtin_amdgcn_ldexp(x, e); }
@@ -116,6 +117,20 @@ __device__ __forceinline__ bool frame_image(const FrameAtom& f, int di, int dj, 
     return fabs(q3) < L.hn && q1 * q1 + q2 * q2 < L.rn2;
 }
 
+// FP64 field of one image of a loop flat to double resolution, evaluated in
+// the loop frame and added in global axes: b fo + o1 e1 + o2 e2 + o3 n.
+__device__ __forceinline__ void add_flat_f64(double q1, double q2, double q3, const LineLoop& L,
+                                             const SegD* __restrict__ segd, double& sx,
+                                             double& sy, double& sz)
+{
+    double fo, o1, o2, o3;
+    line_image_flat_f64<DevMath>(q1, q2, q3, reinterpret_cast<const SegPD*>(segd + L.seg0), L.nv,
+                                 fo, o1, o2, o3);
+    sx += L.bx * fo + ((o1 * L.e1x + o2 * L.e2x) + o3 * L.nx);
+    sy += L.by * fo + ((o1 * L.e1y + o2 * L.e2y) + o3 * L.ny);
+    sz += L.bz * fo + ((o1 * L.e1z + o2 * L.e2z) + o3 * L.nz);
+}
+
 // Range of image indices along one axis that can hold a near image of the
 // loop for this lane: a near image has |q| < sqrt(rn2 + hn^2) < reach, so its
 // offset d L lies within reach of the apex offset pc along every axis.
@@ -132,6 +147,11 @@ __device__ __forceinline__ void near_range(double pc, double iL, double reach, i
 // The three line passes over loops [l0, l1) and images [-img, img]^3 (kk
 // fastest) for the atom at p; loop, segment and image data are wave-uniform.
 enum { PASS_FP64 = 0, PASS_FAR = 1, PASS_NEAR = 2 };
+// PASS_FP64 loop filter (the KIND template argument of that pass): only the
+// loops in global axes (Seg<double>), or only those flat to double resolution
+// (SegPD, the loop-frame walk); each kernel compiles one path, and so needs
+// fewer registers.
+enum { FP64_GLOBAL = 1, FP64_FRAME = 2 };
 
 // FP32 far pass of one flat loop: every lane evaluates every image (uniform
 // control flow keeps the segment table in scalar loads); near lanes discard
@@ -173,11 +193,20 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
     for (int l = l0; l < l1; l++) {
         const LineLoop& L = loops[l];
         const double pcx = px - L.cx, pcy = py - L.cy, pcz = pz - L.cz;
+        FrameAtom f;
         if (PASS == PASS_FP64) {
+            if ((KIND == FP64_GLOBAL && L.flat64) || (KIND == FP64_FRAME && !L.flat64)) continue;
+            if (L.flat64) frame_atom(pcx, pcy, pcz, L, f);
             for (int di = -img; di <= img; di++)
                 for (int dj = -img; dj <= img; dj++)
                     for (int dk = -img; dk <= img; dk++) {
                         if (max(abs(di), max(abs(dj), abs(dk))) <= inner) continue;
+                        if (L.flat64) {
+                            double q1, q2, q3;
+                            frame_image(f, di, dj, dk, L, q1, q2, q3);
+                            add_flat_f64(q1, q2, q3, L, segd, sx, sy, sz);
+                            continue;
+                        }
                         const double qx = pcx - di * Lx, qy = pcy - dj * Ly, qz = pcz - dk * Lz;
                         double ox = 0.0, oy = 0.0, oz = 0.0;
                         line_image_f64<DevMath>(qx, qy, qz, segd + L.seg0, L.nv, L.bx, L.by,
@@ -186,7 +215,6 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
                     }
             continue;
         }
-        FrameAtom f;
         frame_atom(pcx, pcy, pcz, L, f);
         if (PASS == PASS_FAR && KIND != LOOP_3D) {
             const SegP* sp = reinterpret_cast<const SegP*>(segf + L.seg0);
@@ -205,12 +233,16 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
                         if (max(abs(di), max(abs(dj), abs(dk))) <= inner) continue;
                         double q1, q2, q3;
                         if (!frame_image(f, di, dj, dk, L, q1, q2, q3)) continue;
+                        any = true;
+                        if (L.flat64) {
+                            add_flat_f64(q1, q2, q3, L, segd, sx, sy, sz);
+                            continue;
+                        }
                         const double qx = pcx - di * Lx, qy = pcy - dj * Ly, qz = pcz - dk * Lz;
                         double ox = 0.0, oy = 0.0, oz = 0.0;
                         line_image_f64<DevMath>(qx, qy, qz, segd + L.seg0, L.nv, L.bx, L.by,
                                                 L.bz, ox, oy, oz);
                         sx += ox; sy += oy; sz += oz;
-                        any = true;
                     }
             continue;
         }
@@ -655,9 +687,15 @@ int launch(disco_plan* p, int n, const double* x, const double* y, const double*
         const SegF* sf = (const SegF*)p->d_f;
         const SegD* sd = (const SegD*)p->d_d;
         if (p->prec == DISCO_PREC_FP64) {
-            hipLaunchKernelGGL((line_kernel<PASS_FP64, LOOP_3D>), grid, block, 0, s, x, y, z, ux,
-                               uy, uz, n, lp, p->nloops, sf, sd, p->L[0], p->L[1], p->L[2],
+            // loops in global axes, then the loop-frame ones (each kernel
+            // skips the others); the first writes u (or adds, as asked).
+            // Both always run, so a batch sums in the same order.
+            hipLaunchKernelGGL((line_kernel<PASS_FP64, FP64_GLOBAL>), grid, block, 0, s, x, y, z,
+                               ux, uy, uz, n, lp, p->nloops, sf, sd, p->L[0], p->L[1], p->L[2],
                                p->img, p->inner, accumulate);
+            hipLaunchKernelGGL((line_kernel<PASS_FP64, FP64_FRAME>), grid, block, 0, s, x, y, z,
+                               ux, uy, uz, n, lp, p->nloops, sf, sd, p->L[0], p->L[1], p->L[2],
+                               p->img, p->inner, 1);
         } else {
             // far pass: one kernel per loop kind present; the first writes u
             // (or adds, as asked), the others add
@@ -886,9 +924,14 @@ void batch_launch_block(const disco_batch* b, int groups, hipStream_t s, const d
     const dim3 grid((unsigned)groups), block(BLOCK);
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, b->d_counters, 5);
     if (b->prec == DISCO_PREC_FP64) {
-        hipLaunchKernelGGL((batch_line_kernel<BLOCK, PASS_FP64, LOOP_3D>), grid, block, 0, s, x,
-                           y, z, ux, uy, uz, b->d_samples, b->d_items, b->nitems, b->d_counters,
-                           lp, sf, sd, b->img, b->inner, accumulate);
+        // loops in global axes (queue counter 0), then the loop-frame ones
+        // (counter 1), as a single-sample plan sums them
+        hipLaunchKernelGGL((batch_line_kernel<BLOCK, PASS_FP64, FP64_GLOBAL>), grid, block, 0, s,
+                           x, y, z, ux, uy, uz, b->d_samples, b->d_items, b->nitems,
+                           b->d_counters, lp, sf, sd, b->img, b->inner, accumulate);
+        hipLaunchKernelGGL((batch_line_kernel<BLOCK, PASS_FP64, FP64_FRAME>), grid, block, 0, s,
+                           x, y, z, ux, uy, uz, b->d_samples, b->d_items, b->nitems,
+                           b->d_counters + 1, lp, sf, sd, b->img, b->inner, 1);
     } else {
         // far pass: one launch per loop kind present (queue counter k); the
         // first writes u (or adds, as asked) for every sample, the others add

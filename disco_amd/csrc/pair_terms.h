@@ -81,7 +81,8 @@ struct alignas(64) LineLoop {
     float e2f[3];
     float nf[3];
     int seg0, nv;           // first segment in the tables, count
-    int kind, pad0;         // LOOP_3D / LOOP_FLAT / LOOP_GLIDE / LOOP_PRISM
+    int kind;               // LOOP_3D / LOOP_FLAT / LOOP_GLIDE / LOOP_PRISM
+    int flat64;             // flat to double resolution: the FP64 table holds SegPD
     double reach;           // > sqrt(rn2 + hn^2): a near image has its apex this close
     double iLx, iLy, iLz;   // 1 / box lengths (0 for a zero length)
     double pad;
@@ -126,6 +127,18 @@ struct alignas(64) SegP {
 };
 static_assert(sizeof(SegP) == sizeof(Seg<float>), "one FP32 segment slot per segment");
 
+// The same in FP64, for a loop flat to double resolution (LineLoop::flat64):
+// it then takes the FP64 table's slot of the loop instead of Seg<double>.
+struct alignas(128) SegPD {
+    double b1, b2;      // B - c in the plane
+    double t1, t2;      // unit tangent
+    double L;           // |B - A|
+    double n3;          // A1 B2 - A2 B1
+    double w1, w2, w3;  // b x t in the frame
+    double pad[7];
+};
+static_assert(sizeof(SegPD) == sizeof(Seg<double>), "one FP64 segment slot per segment");
+
 // FP32 layout of a loop's segments.  LOOP_3D: Seg<float> in global axes (a
 // loop whose vertices leave its plane by more than float resolution).  The
 // flat kinds use SegP: LOOP_FLAT any b, LOOP_GLIDE b in the plane (b x t is
@@ -149,6 +162,73 @@ DISCO_HD float two_atanh_f(float z)
     return M::log_f((1.0f + z) * M::rcp_f(1.0f - z));
 }
 
+// ------------------------------------------------------------ FP64 elementary functions
+// log and atan2 in double from multiply-adds, one reciprocal and exponent
+// manipulation: far fewer live registers than the library routines, which set
+// the register count (and so the occupancy) of every FP64 line kernel.
+// Accuracy: a few units in the last place (checked against the C library on
+// the host, tests/test_math_host.py).
+
+// 2 atanh(s) = log((1+s)/(1-s)) by its odd series, |s| <= 1/4 (terms through
+// s^27: the first omitted one is below 1e-18 of the sum)
+template <class M>
+DISCO_HD double two_atanh_series_d(double s)
+{
+    const double y = s * s;
+    double p = 1.0 / 27;
+    p = p * y + 1.0 / 25; p = p * y + 1.0 / 23; p = p * y + 1.0 / 21;
+    p = p * y + 1.0 / 19; p = p * y + 1.0 / 17; p = p * y + 1.0 / 15;
+    p = p * y + 1.0 / 13; p = p * y + 1.0 / 11; p = p * y + 1.0 / 9;
+    p = p * y + 1.0 / 7;  p = p * y + 1.0 / 5;  p = p * y + 1.0 / 3;
+    return 2.0 * s + (2.0 * s * y) * p;
+}
+
+// log(w) for finite w >= 1: w = m 2^e with m in [1/sqrt2, sqrt2), log m =
+// 2 atanh((m - 1)/(m + 1)) (|s| <= 0.172), e ln2 in two parts (the high part
+// has 20 trailing zero bits, so e ln2_hi is exact)
+template <class M>
+DISCO_HD double log_ge1_d(double w)
+{
+    int e = M::exponent_d(w);                  // w = f 2^e, f in [1/2, 1)
+    double m = M::ldexp_d(w, -e);
+    if (m < 0.70710678118654752) { m = m * 2.0; e -= 1; }
+    const double sr = (m - 1.0) * M::rcp_d(m + 1.0);
+    const double ln2_hi = 0.6931471804855391, ln2_lo = 7.440617110012397e-11;
+    return e * ln2_hi + (e * ln2_lo + two_atanh_series_d<M>(sr));
+}
+
+// atan2(y, x) with the C library's quadrants and signed zeros for finite
+// arguments: t = min/max of |x|, |y| in [0, 1], reduced about the nearest
+// tan(j pi/16) (j = 0..4) to |u| <= tan(pi/32), then the odd Taylor series of
+// atan(u) through u^17 (first omitted term < 5e-18); atan(tan(j pi/16)) as
+// stored (each table value tau_j is a double, and its arctangent is carried
+// in two parts, computed to 40 digits).
+template <class M>
+DISCO_HD double atan2_own_d(double y, double x)
+{
+    const double ax = M::fabs_d(x), ay = M::fabs_d(y);
+    const bool swap = ay > ax;
+    const double num = swap ? ax : ay, den = swap ? ay : ax;
+    const double t = den > 0.0 ? num * M::rcp_d(den) : 0.0;
+    double tau = 0.0, ahi = 0.0, alo = 0.0;
+    if (t > 0.09849140335716425) { tau = 0.198912367379658; ahi = 0.19634954084936207; alo = 6.846802412842648e-18; }
+    if (t > 0.3033466836073424) { tau = 0.41421356237309503; ahi = 0.39269908169872414; alo = 3.060132146563891e-18; }
+    if (t > 0.5345111359507916) { tau = 0.6681786379192989; ahi = 0.5890486225480862; alo = -5.412106055318847e-18; }
+    if (t > 0.8206787908286602) { tau = 1.0; ahi = 0.7853981633974483; alo = 3.061616997868383e-17; }
+    const double u = (t - tau) * M::rcp_d(1.0 + t * tau);
+    const double v = u * u;
+    double p = -1.0 / 17;
+    p = p * v + 1.0 / 15; p = p * v - 1.0 / 13; p = p * v + 1.0 / 11;
+    p = p * v - 1.0 / 9;  p = p * v + 1.0 / 7;  p = p * v - 1.0 / 5;
+    p = p * v + 1.0 / 3;
+    double a = ahi + (alo + (u - (u * v) * p));
+    const double pi2_hi = 1.5707963267948966, pi2_lo = 6.123233995736766e-17;
+    const double pi_hi = 3.141592653589793, pi_lo = 1.2246467991473532e-16;
+    if (swap) a = pi2_hi + (pi2_lo - a);
+    if (x < 0.0 || (x == 0.0 && M::signbit_d(x))) a = pi_hi + (pi_lo - a);
+    return M::signbit_d(y) ? -a : a;
+}
+
 // 2 atanh(z) in double: for z < 1/4 the odd Taylor series through z^27
 // (first omitted term y^14/29 < 1e-18 relative, y = z^2 <= 1/16), else
 // log1p(2z / (1 - z)).
@@ -164,7 +244,7 @@ DISCO_HD double two_atanh_d(double z)
         p = p * y + 1.0 / 7;  p = p * y + 1.0 / 5;  p = p * y + 1.0 / 3;
         return 2.0 * z + (2.0 * z * y) * p;
     }
-    return M::log1p_d(2.0 * z * M::rcp_d(1.0 - z));
+    return log_ge1_d<M>((1.0 + z) * M::rcp_d(1.0 - z));
 }
 
 // ------------------------------------------------------------ line, FP32
@@ -429,10 +509,127 @@ DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __rest
         ax = px; ay = py; az = pz;
         RA = RB; rA = rB;
     }
-    const double fo = 2.0 * M::atan2_d(Y, X) * kInv4Pi;
+    const double fo = 2.0 * atan2_own_d<M>(Y, X) * kInv4Pi;
     ox += bx * fo + (swx - sax - stx);
     oy += by * fo + (swy - say - sty);
     oz += bz * fo + (swz - saz - stz);
+}
+
+// ------------------------------------------------------------ line, FP64, loop frame
+// line_image_flat_f32's walk in FP64 for a loop flat to double resolution:
+// the atom at (q1, q2, h) in the loop frame, segments as SegPD; the general
+// form (any b).  Reciprocal square roots and reciprocals from the hardware
+// estimates refined by Newton steps; 2 atanh from its series unless a lane of
+// the wave needs the log form.
+struct FlatWalkD {
+    double a1, a2, RA, rA, P;
+    double X, Y;
+    double sw1, sw2, sw3;
+    double x1, x2, sal;
+};
+
+template <class M>
+DISCO_HD double two_atanh_vote_d(double z)
+{
+    const double y = z * z;
+    double p = 1.0 / 27;
+    p = p * y + 1.0 / 25; p = p * y + 1.0 / 23; p = p * y + 1.0 / 21;
+    p = p * y + 1.0 / 19; p = p * y + 1.0 / 17; p = p * y + 1.0 / 15;
+    p = p * y + 1.0 / 13; p = p * y + 1.0 / 11; p = p * y + 1.0 / 9;
+    p = p * y + 1.0 / 7;  p = p * y + 1.0 / 5;  p = p * y + 1.0 / 3;
+    double r = 2.0 * z + (2.0 * z * y) * p;
+    const bool big = !(z < 0.25);
+    if (__builtin_expect(M::any(big), 0)) {                    // a branch, not selects
+        const double l = log_ge1_d<M>((1.0 + z) * M::rcp_d(1.0 - z));
+        r = big ? l : r;
+    }
+    return r;
+}
+
+template <class M>
+DISCO_HD void flat_step_d(const SegPD& g, double q1, double q2, double h2, double u1, double u2,
+                          double uh2, double nh, double mh, FlatWalkD& w)
+{
+    const double p1 = g.b1 - q1, p2 = g.b2 - q2;
+    const double lb2 = p1 * p1 + (p2 * p2 + h2);
+    const double rB = M::rsqrt_d(lb2);
+    const double RB = lb2 * rB;
+    const double rArB = w.rA * rB;
+
+    const double r1b = (uh2 - u1 * p1) - u2 * p2;
+    const double ab = w.a1 * p1 + (w.a2 * p2 + h2);
+    const double den = (RB * w.P + (w.RA * r1b + ab)) * rArB;
+    const double num = (g.n3 * nh) * rArB;
+    w.P = RB + r1b;
+    const double Xn = w.X * den - w.Y * num;
+    w.Y = w.X * num + w.Y * den;
+    w.X = Xn;
+
+    const double sA = w.a1 * g.t1 + w.a2 * g.t2;
+    const double sB = sA + g.L;
+    const double s2 = sA + sB;
+    const double RS = w.RA + RB;
+    double kn = g.L * s2 * rArB;
+    double kd = sB * w.RA + sA * RB;
+    const bool mixed = M::fabs_d(s2) < g.L;                     // sA < 0 < sB
+    if (__builtin_expect(M::any(mixed), 0)) {
+        const double d1 = w.a1 - sA * g.t1, d2 = w.a2 - sA * g.t2;
+        const double dd = d1 * d1 + (d2 * d2 + h2);
+        const double knm = sB * rB - sA * w.rA;
+        kd = mixed ? dd : kd;
+        kn = mixed ? knm : kn;
+    }
+    const double rr = M::rcp_d(RS * kd);
+    const double J0 = two_atanh_vote_d<M>(g.L * kd * rr);
+    const double K0 = kn * RS * rr;
+    const double K1 = w.rA - rB;
+    const double dw = w.a1 * g.w1 + (w.a2 * g.w2 + mh * g.w3);
+    const double al = dw * K0;
+    const double be = dw * (K1 - sA * K0);
+    w.sw1 += J0 * g.w1;
+    w.sw2 += J0 * g.w2;
+    w.sw3 += J0 * g.w3;
+    w.x1 = (w.x1 + al * w.a1) + be * g.t1;
+    w.x2 = (w.x2 + al * w.a2) + be * g.t2;
+    w.sal += al;
+
+    w.a1 = p1; w.a2 = p2;
+    w.RA = RB; w.rA = rB;
+}
+
+// Field of one image in the loop frame: fo = Omega / 4 pi (the caller adds
+// b fo) and o = cJ sum J0 w - cK sum (al a + be t).
+template <class M>
+DISCO_HD void line_image_flat_f64(double q1, double q2, double h, const SegPD* __restrict__ s,
+                                  int nv, double& fo, double& o1, double& o2, double& o3)
+{
+    const double h2 = h * h;
+    const double rl1 = M::rsqrt_d(q1 * q1 + (q2 * q2 + h2));
+    const double u1 = q1 * rl1, u2 = q2 * rl1, uh2 = h2 * rl1, nh = -h * rl1, mh = -h;
+
+    FlatWalkD w;
+    const SegPD& last = s[nv - 1];
+    w.a1 = last.b1 - q1;
+    w.a2 = last.b2 - q2;
+    const double la2 = w.a1 * w.a1 + (w.a2 * w.a2 + h2);
+    w.rA = M::rsqrt_d(la2);
+    w.RA = la2 * w.rA;
+    w.P = w.RA + (uh2 - (u1 * w.a1 + u2 * w.a2));
+    w.X = 1.0; w.Y = 0.0;
+    w.sw1 = w.sw2 = w.sw3 = 0.0;
+    w.x1 = w.x2 = w.sal = 0.0;
+    for (int k0 = 0; k0 < nv; k0 += 64) {
+        const int k1 = nv < k0 + 64 ? nv : k0 + 64;
+#pragma unroll 1
+        for (int k = k0; k < k1; k++) flat_step_d<M>(s[k], q1, q2, h2, u1, u2, uh2, nh, mh, w);
+        const int e = M::exponent_d(M::fmax_d(M::fabs_d(w.X), M::fabs_d(w.Y)));
+        w.X = M::ldexp_d(w.X, -e);
+        w.Y = M::ldexp_d(w.Y, -e);
+    }
+    fo = 2.0 * atan2_own_d<M>(w.Y, w.X) * kInv4Pi;
+    o1 = kCJ * w.sw1 - kC3 * w.x1;
+    o2 = kCJ * w.sw2 - kC3 * w.x2;
+    o3 = kCJ * w.sw3 - kC3 * (mh * w.sal);
 }
 
 // ------------------------------------------------------------ loopons

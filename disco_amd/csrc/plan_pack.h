@@ -134,6 +134,16 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, d
             const int j = (k + nv - 1) % nv;
             flat = std::hypot(p1[k] - p1[j], p2[k] - p2[j]) > res;
         }
+        // FP64 layout: the frame form (SegPD) only when the vertices sit in the
+        // plane to double resolution (every DISCO shape: heights ~1e-16 of its
+        // size); otherwise Seg<double> in global axes
+        const double res64 = std::ldexp(std::fmax(rmax, 1.0), -48);
+        bool flat64 = hmax <= res64;
+        for (int k = 0; k < nv && flat64; k++) {
+            const int j = (k + nv - 1) % nv;
+            flat64 = std::hypot(p1[k] - p1[j], p2[k] - p2[j]) > res64;
+        }
+        L.flat64 = flat64 ? 1 : 0;
         const double b1 = lp.b[0] * L.e1x + lp.b[1] * L.e1y + lp.b[2] * L.e1z;
         const double b2 = lp.b[0] * L.e2x + lp.b[1] * L.e2y + lp.b[2] * L.e2z;
         const double b3 = lp.b[0] * nx + lp.b[1] * ny + lp.b[2] * nz;
@@ -201,6 +211,20 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, d
                 g.w2 = (float)(fb3 * t1);
                 g.w3 = (float)(fb1 * t2 - fb2 * t1);
                 std::memcpy(&f, &g, sizeof(f));
+            }
+            if (flat64) {
+                SegPD e;
+                std::memset(&e, 0, sizeof(e));
+                const double e1 = p1[k] - p1[j], e2 = p2[k] - p2[j];
+                const double len2 = std::hypot(e1, e2);
+                e.b1 = p1[k]; e.b2 = p2[k];
+                e.t1 = e1 / len2; e.t2 = e2 / len2;
+                e.L = len2;
+                e.n3 = p1[j] * p2[k] - p2[j] * p1[k];
+                e.w1 = -b3 * e.t2;
+                e.w2 = b3 * e.t1;
+                e.w3 = b1 * e.t2 - b2 * e.t1;
+                std::memcpy(&d, &e, sizeof(d));
             }
             out.segd.push_back(d);
             out.segf.push_back(f);
