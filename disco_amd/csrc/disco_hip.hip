@@ -424,6 +424,48 @@ __global__ __launch_bounds__(kBlock) void loopon_kernel(
     ux[i] = ax; uy[i] = ay; uz[i] = az;
 }
 
+// A loopon plan over few atoms (the nodes of a coarse grid): the loops split
+// into `split` contiguous groups along the grid's y dimension, so the launch
+// has split times as many workgroups; each group's sum goes to its own slice of
+// `partial`, and loopon_split_sum_kernel adds the slices in group order.
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void loopon_split_kernel(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    int n, const LooponLoop* __restrict__ loops, int nloops, const float4* __restrict__ pf,
+    const double4* __restrict__ pd, double Lx, double Ly, double Lz, int img, int inner,
+    int split, double* __restrict__ partial)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int g = blockIdx.y;
+    const int l0 = (int)((long long)g * nloops / split);
+    const int l1 = (int)((long long)(g + 1) * nloops / split);
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    atom_loopon<PREC>(x[i], y[i], z[i], loops, l0, l1, pf, pd, Lx, Ly, Lz, img, inner, ax, ay,
+                      az);
+    double* p = partial + (size_t)g * 3 * (size_t)n;
+    p[i] = ax;
+    p[(size_t)n + i] = ay;
+    p[2 * (size_t)n + i] = az;
+}
+
+__global__ __launch_bounds__(kBlock) void loopon_split_sum_kernel(
+    const double* __restrict__ partial, int n, int split, double* __restrict__ ux,
+    double* __restrict__ uy, double* __restrict__ uz, int accumulate)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int g = 0; g < split; g++) {
+        const double* p = partial + (size_t)g * 3 * (size_t)n;
+        ax += p[i];
+        ay += p[(size_t)n + i];
+        az += p[2 * (size_t)n + i];
+    }
+    if (accumulate) { ax += ux[i]; ay += uy[i]; az += uz[i]; }
+    ux[i] = ax; uy[i] = ay; uz[i] = az;
+}
+
 // u = 0 over [0, n): clears outputs without a memset node (graph-capture safe)
 __global__ __launch_bounds__(kBlock) void zero3_kernel(double* __restrict__ a,
                                                        double* __restrict__ b,
@@ -587,6 +629,9 @@ struct disco_plan {
     double L[3] = {0, 0, 0};
     double work = 0;       // segments (or loopons) per image
     int kofs[5] = {0, 0, 0, 0, 0};   // line plans: loops of kind k are [kofs[k], kofs[k+1])
+    int split = 1;                   // loopon plans: loop groups per launch (few atoms)
+    double* d_split = nullptr;       // their partial sums (3 n split doubles, grow-only)
+    size_t cap_split = 0;
     void* d_loops = nullptr;
     void* d_f = nullptr;   // SegF / float4 table
     void* d_d = nullptr;   // SegD / double4 table
@@ -665,6 +710,7 @@ void free_plan(disco_plan* p)
     if (p->d_loops) (void)hipFree(p->d_loops);
     if (p->d_f) (void)hipFree(p->d_f);
     if (p->d_d) (void)hipFree(p->d_d);
+    if (p->d_split) (void)hipFree(p->d_split);
     delete p;
 }
 
@@ -719,7 +765,29 @@ int launch(disco_plan* p, int n, const double* x, const double* y, const double*
         const LooponLoop* lp = (const LooponLoop*)p->d_loops;
         const float4* pf = (const float4*)p->d_f;
         const double4* pd = (const double4*)p->d_d;
-        if (p->prec == DISCO_PREC_FP64)
+        const int split = p->split < p->nloops ? p->split : p->nloops;
+        if (split > 1) {
+            const size_t need = (size_t)3 * (size_t)n * (size_t)split * sizeof(double);
+            if (p->cap_split < need) {
+                if (p->d_split) DISCO_HIP_TRY(ctx, hipFree(p->d_split));
+                p->d_split = nullptr;
+                p->cap_split = 0;
+                DISCO_HIP_TRY(ctx, hipMalloc(&p->d_split, need));
+                p->cap_split = need;
+            }
+            const dim3 g2(grid.x, (unsigned)split);
+            if (p->prec == DISCO_PREC_FP64)
+                hipLaunchKernelGGL(loopon_split_kernel<DISCO_PREC_FP64>, g2, block, 0, s, x, y, z,
+                                   n, lp, p->nloops, pf, pd, p->L[0], p->L[1], p->L[2], p->img,
+                                   p->inner, split, p->d_split);
+            else
+                hipLaunchKernelGGL(loopon_split_kernel<DISCO_PREC_FP32C>, g2, block, 0, s, x, y,
+                                   z, n, lp, p->nloops, pf, pd, p->L[0], p->L[1], p->L[2], p->img,
+                                   p->inner, split, p->d_split);
+            DISCO_HIP_TRY(ctx, hipGetLastError());
+            hipLaunchKernelGGL(loopon_split_sum_kernel, grid, block, 0, s, p->d_split, n, split,
+                               ux, uy, uz, accumulate);
+        } else if (p->prec == DISCO_PREC_FP64)
             hipLaunchKernelGGL(loopon_kernel<DISCO_PREC_FP64>, grid, block, 0, s, x, y, z, ux, uy,
                                uz, n, lp, p->nloops, pf, pd, p->L[0], p->L[1], p->L[2], p->img,
                                p->inner, accumulate);
@@ -1247,6 +1315,15 @@ int disco_hip_plan_set_inner(disco_plan* plan, int inner)
     if (!plan) return DISCO_ERR_ARG;
     if (inner < -1 || inner > plan->img) return fail(plan->ctx, DISCO_ERR_ARG, "inner shell out of range");
     plan->inner = inner;
+    return DISCO_OK;
+}
+
+int disco_hip_plan_set_split(disco_plan* plan, int groups)
+{
+    if (!plan) return DISCO_ERR_ARG;
+    if (plan->kind != 1) return fail(plan->ctx, DISCO_ERR_ARG, "loop groups are for loopon plans");
+    if (groups < 1 || groups > 256) return fail(plan->ctx, DISCO_ERR_ARG, "groups out of [1, 256]");
+    plan->split = groups;
     return DISCO_OK;
 }
 

@@ -73,6 +73,7 @@ def _lib():
                                                 ctypes.POINTER(_SLoop), ctypes.c_int, _dp,
                                                 ctypes.c_int, _vp]
         L.disco_hip_plan_set_inner.argtypes = [_vp, ctypes.c_int]
+        L.disco_hip_plan_set_split.argtypes = [_vp, ctypes.c_int]
         L.disco_hip_batch_set_inner.argtypes = [_vp, ctypes.c_int]
         _bound = True
     return L
@@ -292,6 +293,11 @@ def set_inner(plan: B.Plan, inner: int):
     plan.ctx.check(_lib().disco_hip_plan_set_inner(plan._h, int(inner)))
 
 
+def set_split(plan: B.Plan, groups: int):
+    """Loop groups per launch of a loopon plan evaluated at few atoms."""
+    plan.ctx.check(_lib().disco_hip_plan_set_split(plan._h, int(groups)))
+
+
 def exact_far_shell_torch(ctx, x, y, z, loops, box, img, prec=B.PREC_FP32C):
     """Exact far shell (images with max index in (1, img]) of line loops."""
     import torch
@@ -321,7 +327,7 @@ def grid_nodes(x, y, z, h):
 
 
 def base_far_shell_torch(ctx, x, y, z, loops, box, img, kind="rule7", u=None,
-                         accumulate=False, prec=B.PREC_FP32C, grid_h=None, grid_groups=8):
+                         accumulate=False, prec=B.PREC_FP32C, grid_h=None, grid_groups=None):
     """Far shell of the loops' loopon rule (`kind`), summed by the loopon kernels:
     at every atom, or (grid_h set) at the nodes of a grid of that spacing over
     the atoms and interpolated to them -- the far shell is smooth on the scale of
@@ -338,29 +344,21 @@ def base_far_shell_torch(ctx, x, y, z, loops, box, img, kind="rule7", u=None,
         return u
     origin, dims, nodes = grid_nodes(x, y, z, grid_h)
     # a grid holds few nodes (tens of thousands) for the GPU's 256 CUs: the loops
-    # are split into groups summed concurrently on their own streams, then added
-    # in group order
-    groups = max(1, min(grid_groups, len(base)))
-    main = torch.cuda.current_stream()
-    plans = []
-    for g in range(groups):                 # every upload before the first launch
-        plan = B.Plan(ctx, base[g::groups], box, img, prec, kind="loopon")
-        set_inner(plan, 1)
-        plans.append(plan)
-    parts = [[torch.zeros_like(nodes[0]) for _ in range(3)] for _ in plans]
-    sides = [torch.cuda.Stream(device=x.device) for _ in plans]
-    for plan, out, side in zip(plans, parts, sides):
-        side.wait_stream(main)
-        plan.run_torch(*nodes, *out, stream=side.cuda_stream)
-    for side in sides:
-        main.wait_stream(side)
-    gu = [sum(p[k] for p in parts) for k in range(3)]
+    # are split into contiguous groups, each summed by its own workgroups of one
+    # launch, then added in group order (about 2,048 workgroups in all)
+    tiles = (nodes[0].numel() + 255) // 256
+    want = grid_groups if grid_groups else -(-2048 // tiles)
+    groups = max(1, min(want, len(base), 256))
+    plan = B.Plan(ctx, base, box, img, prec, kind="loopon")
+    set_inner(plan, 1)
+    set_split(plan, groups)
+    gu = [torch.zeros_like(nodes[0]) for _ in range(3)]
+    plan.run_torch(*nodes, *gu)
     shape = (dims[2], dims[1], dims[0])
     B.grid_interp_torch(ctx, x, y, z, *u, [t.view(shape) for t in gu], origin, grid_h,
                         accumulate=accumulate)
-    torch.cuda.synchronize()            # the plans' tables are freed below
-    for plan in plans:
-        plan.free()
+    torch.cuda.synchronize()            # the plan's tables are freed below
+    plan.free()
     return u
 
 

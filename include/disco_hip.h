@@ -150,6 +150,11 @@ int disco_hip_plan_kernel_ms(disco_plan *plan, double *ms, int nms);
  * are skipped (inner = -1, the default, keeps them all; inner <= img).  A
  * far-shell plan (inner = 1) is what the surrogate replaces and validates. */
 int disco_hip_plan_set_inner(disco_plan *plan, int inner);
+/* (No counterpart.) A loopon plan evaluated at few atoms (say the nodes of a
+ * coarse grid) splits its loops into `groups` contiguous groups summed by
+ * separate workgroups of one launch, then added in group order (1 = off,
+ * <= 256); scratch of 24 B x atoms x groups is kept with the plan. */
+int disco_hip_plan_set_split(disco_plan *plan, int groups);
 /* Work of one run per atom: segments (or loopons) x images. */
 double disco_hip_plan_evals_per_atom(const disco_plan *plan);
 int disco_hip_plan_free(disco_plan *plan);

@@ -116,3 +116,28 @@ def test_loopon_forms_vs_oracle(hip_ctx, form):
         err = np.abs(u - ref).max()
         print(f"{form} prec {prec}: {err:.3e}  |u|max {np.abs(ref).max():.2f}")
         assert err < tol
+
+
+@pytest.mark.parametrize("prec", [B.PREC_FP64, B.PREC_FP32C])
+def test_loop_groups_match_one_launch(hip_ctx, prec):
+    """A loopon plan split into loop groups (one launch, partial sums added in
+    group order) gives the single-launch sum to rounding, for group counts that
+    do and do not divide the loop count; the inner-shell setting carries over."""
+    import torch
+    from disco_amd import surrogate as S
+    c3 = G.config_c3(nloops=7)
+    idx = np.arange(0, c3.n, 97)[:3000]
+    dev = torch.device("cuda:0")
+    x, y, z = (torch.from_numpy(np.ascontiguousarray(a[idx])).to(dev) for a in (c3.x, c3.y, c3.z))
+    plan = B.Plan(hip_ctx, c3.loops, c3.box, 2, prec, kind="loopon")
+    S.set_inner(plan, 1)
+    ref = [torch.zeros_like(x) for _ in range(3)]
+    plan.run_torch(x, y, z, *ref)
+    scale = max(t.abs().max().item() for t in ref)
+    for groups in (2, 3, 7, 16):
+        S.set_split(plan, groups)
+        got = [torch.full_like(x, 1.0) for _ in range(3)]
+        plan.run_torch(x, y, z, *got, accumulate=True)       # adds to the ones
+        err = max((g - 1.0 - r).abs().max().item() for g, r in zip(got, ref))
+        assert err <= 1e-12 * max(scale, 1.0), (groups, err)
+    plan.free()
