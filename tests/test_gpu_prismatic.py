@@ -11,7 +11,7 @@ from conftest import GOLDEN, golden
 from disco_amd import backend as B
 from disco_amd import geometry as G
 from disco_amd import prismatic as P
-from test_oracle_prismatic import random_prismatic_loops
+from test_oracle_prismatic import REF_CASES, hexagon, random_prismatic_loops
 
 sys.path.insert(0, GOLDEN)
 import make_prismatic as MP  # noqa: E402
@@ -104,3 +104,39 @@ def test_slab_edge_cases(hip_ctx):
     empty = torch.empty(0, dtype=torch.float64, device="cuda")
     a, s, info = P.prismatic_slab_torch(hip_ctx, vac, sp, empty, empty, empty, box)
     assert info["count"] == 0
+
+
+@pytest.mark.parametrize("case", range(len(REF_CASES)))
+def test_reference_unit_test_slabs(hip_ctx, case):
+    """The reference test's checks on the device slab: matter type, |b.n| / d
+    planes, slab size within 15 % of rho A |b.n|, every atom inside the slab
+    bounds on its reported side -- and the slab equal to the oracle's."""
+    kind, shape, m, n, rc, vdir, expect = REF_CASES[case]
+    a0, N = 3.3024, 24
+    x, y, z = B.lattice_torch(hip_ctx, kind, N, N, N, a0)
+    c, d = G.native_snap_to_mid_plane([1.1, -0.7, 0.4], n, a0, N, N, N, kind)
+    R = 18.0
+    if shape == "circle":
+        lp = G.Loop("circle", c, m, n, a0, radius=R)
+        area = np.pi * R * R
+    else:
+        lp = G.Loop("polygon", c, m, n, a0, vertices=hexagon(R, m, n, vdir), corner_r=rc,
+                    axis=vdir or [])
+        area = 1.5 * np.sqrt(3) * R * R - rc * rc * (6 / np.sqrt(3) - np.pi)
+    st = G.native_loop_state(lp)
+    box = [N * a0] * 3
+    atoms, side, info = P.prismatic_slab_torch(hip_ctx, st, d, x, y, z, box)
+    atoms, side = atoms.cpu().numpy(), side.cpu().numpy()
+    bn = abs(np.dot(st.bmag * np.array(st.m[:]), np.array(st.n[:])))
+    assert info["type"] == expect == P.matter_type(st)
+    assert info["planes"] == round(bn / d)
+    rho = (2 if kind == "BCC" else 4) / a0 ** 3
+    nominal = rho * area * bn
+    assert abs(atoms.size - nominal) < 0.15 * nominal, (atoms.size, nominal)
+    hx, hy, hz = (t.cpu().numpy() for t in (x, y, z))
+    s = ((hx[atoms] - c[0]) * st.n[0] + (hy[atoms] - c[1]) * st.n[1] + (hz[atoms] - c[2]) * st.n[2])
+    t_lo, t_hi = -np.floor(info["planes"] / 2), np.ceil(info["planes"] / 2)
+    assert np.all(s / d >= t_lo - 1e-9) and np.all(s / d < t_hi)
+    assert np.array_equal(s > 0, side > 0)
+    oa, os_, _ = oracle.prismatic_slab(st, d, hx, hy, hz, box)
+    assert np.array_equal(atoms, oa) and np.array_equal(side, os_)

@@ -133,3 +133,71 @@ def test_host_matter_type_matches_oracle():
     c = [0.0, 0.0, 0.0]
     glide = G.native_loop_state(G.Loop("circle", c, [1, -1, 0], [1, 1, 1], 2.8553, radius=5.0))
     assert P.matter_type(glide) == oracle.matter_type(glide) == 0
+
+
+def hexagon(R, m, n, vdir):
+    """Regular hexagon offsets (first vertex along vdir projected on the plane,
+    or the default axis), counter-clockwise about n."""
+    nh = np.asarray(n, float) / np.linalg.norm(n)
+    if vdir:
+        u = np.asarray(vdir, float)
+        u = u - u.dot(nh) * nh
+    else:
+        u = np.asarray(G.default_axes(G._unit(m), G._unit(n))[0], float)
+    u /= np.linalg.norm(u)
+    v = np.cross(u, nh)
+    return [list(R * (np.cos(t) * u + np.sin(t) * v)) for t in np.arange(6) * np.pi / 3]
+
+
+# The slab cases of the reference's own unit test (Code/tests/test_prismatic.cpp:
+# 224-245): A0 = 3.3024, 24^3 cells, centre snapped from (1.1, -0.7, 0.4),
+# circles and hexagons of R = 18 A (corner radius 0 or 3, vertex axis <1-10> or
+# default); (lattice, shape, m, n, corner, vertex axis, expected type)
+T3 = 2.0 / 3.0
+REF_CASES = [
+    ("BCC", "circle", [1, 1, 1], [1, 1, 1], 0, None, 1),
+    ("BCC", "circle", [-1, -1, -1], [1, 1, 1], 0, None, 2),
+    ("BCC", "circle", [1, -1, 1], [1, -1, 0], 0, None, 1),
+    ("BCC", "circle", [-1, 1, -1], [1, -1, 0], 0, None, 2),
+    ("BCC", "circle", [0, 0, 2], [0, 0, 1], 0, None, 1),
+    ("BCC", "hexagon", [-1, -1, -1], [1, 1, 1], 0.0, [1, -1, 0], 2),
+    ("BCC", "hexagon", [1, 1, 1], [1, 1, 1], 3.0, [1, -1, 0], 1),
+    ("BCC", "hexagon", [-1, -1, -1], [1, 1, 1], 3.0, None, 2),
+    ("FCC", "circle", [1, 1, 0], [1, 1, 0], 0, None, 1),
+    ("FCC", "circle", [-1, -1, 0], [1, 1, 0], 0, None, 2),
+    ("FCC", "circle", [1, 0, 1], [1, 1, 1], 0, None, 1),
+    ("FCC", "circle", [-1, 0, -1], [1, 1, 1], 0, None, 2),
+    ("FCC", "circle", [0, 0, 2], [0, 0, 1], 0, None, 1),
+    ("FCC", "hexagon", [-1, -1, 0], [1, 1, 0], 3.0, None, 2),
+    ("FCC", "circle", [T3, T3, T3], [1, 1, 1], 0, None, 1),          # Frank, vacancy
+    ("FCC", "circle", [-T3, -T3, -T3], [1, 1, 1], 0, None, 2),       # Frank, interstitial
+]
+
+
+@pytest.mark.parametrize("case", range(len(REF_CASES)))
+def test_oracle_passes_the_reference_unit_test_checks(case):
+    """The oracle's slab for each case of Code/tests/test_prismatic.cpp satisfies
+    that test's checks: type, |b.n| / d planes, size within 15 % of rho A |b.n|,
+    atoms inside the slab bounds on their reported side."""
+    kind, shape, m, n, rc, vdir, expect = REF_CASES[case]
+    a0, N = 3.3024, 24
+    x, y, z = G.lattice(kind, N, N, N, a0)
+    c, d = G.native_snap_to_mid_plane([1.1, -0.7, 0.4], n, a0, N, N, N, kind)
+    R = 18.0
+    if shape == "circle":
+        lp = G.Loop("circle", c, m, n, a0, radius=R)
+        area = np.pi * R * R
+    else:
+        lp = G.Loop("polygon", c, m, n, a0, vertices=hexagon(R, m, n, vdir), corner_r=rc,
+                    axis=vdir or [])
+        area = 1.5 * np.sqrt(3) * R * R - rc * rc * (6 / np.sqrt(3) - np.pi)
+    st = G.native_loop_state(lp)
+    atoms, side, info = oracle.prismatic_slab(st, d, x, y, z, [N * a0] * 3)
+    bn = abs(np.dot(st.bmag * np.array(st.m[:]), np.array(st.n[:])))
+    assert int(info[0]) == expect == oracle.matter_type(st)
+    assert int(info[1]) == round(bn / d)
+    nominal = (2 if kind == "BCC" else 4) / a0 ** 3 * area * bn
+    assert abs(atoms.size - nominal) < 0.15 * nominal
+    s = (x[atoms] - c[0]) * st.n[0] + (y[atoms] - c[1]) * st.n[1] + (z[atoms] - c[2]) * st.n[2]
+    assert np.all(s / d >= -np.floor(info[1] / 2) - 1e-9) and np.all(s / d < np.ceil(info[1] / 2))
+    assert np.array_equal(s > 0, side > 0)
