@@ -184,3 +184,27 @@ def test_native_rejects_bad_input():
         G.native_snap_to_mid_plane([0, 0, 0], [0.5, 1, 0], 2.9, 10, 10, 10, "BCC")
     with pytest.raises(ValueError):
         G.native_snap_to_mid_plane([0, 0, 0], [0, 0, 0], 2.9, 10, 10, 10, "BCC")
+
+
+@pytest.mark.parametrize("dims", [(12, 12, 12), (11, 13, 9)])
+@pytest.mark.parametrize("kind", ["BCC", "FCC"])
+def test_snap_reference_unit_test(dims, kind):
+    """Code/tests/test_snap.cpp on the producer: the snapped centre moves along n
+    only, the reported spacing is the distance between the lattice's planes
+    around it (brute force over the sites), and it sits on their mid-plane."""
+    a0, c = 3.3024, [1.3, -2.7, 0.9]
+    x, y, z = G.lattice(kind, *dims, a0)
+    for n in ([1, 1, 0], [1, -1, 2], [1, 2, -3], [1, 0, 0], [2, 2, 0], [1, 1, 1], [1, -1, 3]):
+        s, d = G.native_snap_to_mid_plane(c, n, a0, *dims, kind)
+        nh = np.asarray(n, float) / np.linalg.norm(n)
+        ps, pc = float(np.dot(s, nh)), float(np.dot(c, nh))
+        assert np.abs((np.array(s) - c) - (ps - pc) * nh).max() < 1e-9
+        p = x * nh[0] + y * nh[1] + z * nh[2]
+        below, above = p[p <= ps].max(), p[p > ps].min()
+        assert abs((above - below) - d) < 1e-9
+        assert abs((ps - below) - d / 2) < 1e-9
+
+
+def test_snap_centre_on_a_plane_goes_to_the_plus_side():
+    s, d = G.native_snap_to_mid_plane([0.0, 0.0, 0.0], [1, 1, 0], 3.3024, 12, 12, 12, "BCC")
+    assert abs((s[0] + s[1]) / np.sqrt(2.0) - d / 2) < 1e-9
