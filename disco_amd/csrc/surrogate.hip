@@ -11,7 +11,8 @@
 //     dipole scales with the fourth power of its size over the distance).
 // Layer 1 runs in FP32 on the VALU (the atom offset as a BF16 input would be
 // quantised to ~L/256).  Layers 2-4 are BF16 GEMMs on MFMA
-// (v_mfma_f32_32x32x8_bf16) with FP32 accumulation; FP32 master weights are
+// (v_mfma_f32_32x32x16_bf16; the split-K weight gradients on
+// v_mfma_f32_32x32x8_bf16) with FP32 accumulation; FP32 master weights are
 // updated by Adam; all of it by this file's kernels.
 #include <hip/hip_runtime.h>
 
@@ -33,6 +34,8 @@ namespace {
 
 typedef uint16_t bf16;
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int kIn = 16;    // 14 features + 2 zero pads
@@ -135,7 +138,8 @@ __global__ void layer1_kernel(const float* __restrict__ X, const float* __restri
 
 // C (M x N) = A (M x K) . B (K x N) on MFMA, BF16 operands, FP32 accumulate.
 // A(i, k) = A[i*sam + k*sak], B(k, j) = B[k*sbk + j*sbn].  One wave64 per
-// 32 x 32 tile; grid (ceil(N/32), ceil(M/32)); out-of-range loads read zero
+// 32 x 32 tile, K16 steps with the k of each operand slot as in the fused
+// apply (so the forward and the apply round the same sums alike); grid (ceil(N/32), ceil(M/32)); out-of-range loads read zero
 // and out-of-range outputs are not stored.
 // EPI 0: C = acc (+ bias)                (fp32 out)
 // EPI 1: Z = acc + bias, A = bf16(silu)  (fp32 Z may be null, bf16 Ab)
@@ -151,14 +155,15 @@ __global__ __launch_bounds__(64) void gemm_bf16_kernel(
     const int kq = 4 * (lane >> 5);
     v16f acc;
     for (int t = 0; t < 16; t++) acc[t] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += 8) {
-        v4s a, b;
-        for (int t = 0; t < 4; t++) {
-            const int k = k0 + kq + t;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        v8s a, b;
+        for (int t = 0; t < 8; t++) {
+            const int k = k0 + kq + (t & 3) + 8 * (t >> 2);   // the fused apply's k order
             a[t] = (ia < M && k < K) ? (short)A[ia * sam + k * sak] : (short)0;
             b[t] = (jb < N && k < K) ? (short)B[k * sbk + jb * sbn] : (short)0;
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, a),
+                                                      __builtin_bit_cast(v8bf, b), acc, 0, 0, 0);
     }
     const long long jc = j0 + (lane & 31);
     if (jc >= N) return;
@@ -400,37 +405,51 @@ __global__ void loop_consts_kernel(const float* __restrict__ P,
     if (j == 0) sl[l] = (float)(gain * pair_scale(loops[l], L));
 }
 
-// LDS rows of the BF16 weights are padded to 132 entries (264 B = 66 banks),
-// so the 32 rows an MFMA operand read touches fall on distinct bank pairs.
-constexpr int kLd = kH + 4;
+// The fused apply runs its layers on v_mfma_f32_32x32x16_bf16: 8 bf16 of k per
+// lane, k' = 8 (lane / 32) + i.  The B operand (activations) of one K16 step is
+// two K8 groups of the previous accumulator, neurons 16t + 4h + i and
+// 16t + 8 + 4h + i (h = lane / 32, i < 4); the weights are stored in LDS with
+// bits 2 and 3 of the column swapped, so the matching A operand is the 16
+// contiguous bytes at 16t + 8h.  Rows are padded to 136 entries (272 B = 68
+// banks): the 8 lanes of one 128-bit read phase start 4 banks apart.
+constexpr int kLd = kH + 8;
+__device__ __forceinline__ int k16_col(int c) { return (c & ~12) | ((c & 4) << 1) | ((c & 8) >> 1); }
 
-// One hidden layer of the fused apply: hb (the B operands of its 16 k-steps)
+__device__ __forceinline__ v16f mfma_k16(const bf16* __restrict__ wrow, const v8s& b, v16f acc)
+{
+    const v8s a = *reinterpret_cast<const v8s*>(wrow);
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, a),
+                                                   __builtin_bit_cast(v8bf, b), acc, 0, 0, 0);
+}
+
+// One hidden layer of the fused apply: hb (the B operands of its 8 k-steps)
 // is replaced by the layer's output in the same layout.
 __device__ __forceinline__ void hidden_layer(const bf16* __restrict__ W,
                                              const float* __restrict__ bias, int col, int h,
-                                             v4s (&hb)[16])
+                                             v8s (&hb)[8])
 {
-    v4s hn[16];
+    v8s hn[8];
 #pragma unroll
     for (int np = 0; np < 4; np += 2) {
         v16f acc0, acc1;
         for (int t = 0; t < 16; t++) { acc0[t] = 0.0f; acc1[t] = 0.0f; }
 #pragma unroll
-        for (int s = 0; s < 16; s++) {
-            const v4s w0 = *reinterpret_cast<const v4s*>(W + (32 * np + col) * kLd + 8 * s + 4 * h);
-            const v4s w1 = *reinterpret_cast<const v4s*>(W + (32 * np + 32 + col) * kLd + 8 * s + 4 * h);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w0, hb[s], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w1, hb[s], acc1, 0, 0, 0);
+        for (int s = 0; s < 8; s++) {
+            acc0 = mfma_k16(W + (32 * np + col) * kLd + 16 * s + 8 * h, hb[s], acc0);
+            acc1 = mfma_k16(W + (32 * np + 32 + col) * kLd + 16 * s + 8 * h, hb[s], acc1);
         }
+        // register t holds neuron 8(t/4) + 4h + t%4 of the tile: K16 group
+        // 2np + t/8 (+2 for the second tile), half (t/4)&1, element t%4
 #pragma unroll
         for (int t = 0; t < 16; t++) {
             const int n0 = 32 * np + 8 * (t >> 2) + 4 * h + (t & 3);
-            hn[4 * np + (t >> 2)][t & 3] = (short)f2bf_d(siluf_(acc0[t] + bias[n0]));
-            hn[4 * np + 4 + (t >> 2)][t & 3] = (short)f2bf_d(siluf_(acc1[t] + bias[n0 + 32]));
+            const int g = 2 * np + (t >> 3), e = 4 * ((t >> 2) & 1) + (t & 3);
+            hn[g][e] = (short)f2bf_d(siluf_(acc0[t] + bias[n0]));
+            hn[g + 2][e] = (short)f2bf_d(siluf_(acc1[t] + bias[n0 + 32]));
         }
     }
 #pragma unroll
-    for (int s = 0; s < 16; s++) hb[s] = hn[s];
+    for (int s = 0; s < 8; s++) hb[s] = hn[s];
 }
 
 // u[a] += sum over loops of scale_l * MLP(features(a, l)), the whole network in
@@ -438,7 +457,7 @@ __device__ __forceinline__ void hidden_layer(const bf16* __restrict__ W,
 // each layer is computed transposed, out^T = W . h^T, with the weights as the
 // MFMA A operand (from LDS) and the activations as B: the accumulator of one
 // 32-neuron tile (lane: its column; register t: neuron 8(t/4) + 4(lane/32) +
-// t%4) is exactly the B operand of the next layer's four k-steps, so
+// t%4) is exactly the B operand of the next layer's two K16 steps, so
 // activations never leave registers.  Layer 1 (three offset features; the
 // loop's features are in cl) runs on the VALU in FP32.
 __global__ __launch_bounds__(256) void apply_fused_kernel(
@@ -453,9 +472,10 @@ __global__ __launch_bounds__(256) void apply_fused_kernel(
     __shared__ float sb2[kH], sb3[kH], sb4[kOut];
     for (int i = threadIdx.x; i < kH * kH; i += 256) {
         const int r = i / kH, c = i % kH;
-        sW2[r * kLd + c] = Wb[i];
-        sW3[r * kLd + c] = Wb[kH * kH + i];
-        if (r < kOut) sW4[r * kLd + c] = Wb[2 * kH * kH + i];
+        const int p = r * kLd + k16_col(c);
+        sW2[p] = Wb[i];
+        sW3[p] = Wb[kH * kH + i];
+        if (r < kOut) sW4[p] = Wb[2 * kH * kH + i];
     }
     for (int i = threadIdx.x; i < kH; i += 256) {
         const float* w = P + oW1 + i * kIn;
@@ -480,29 +500,32 @@ __global__ __launch_bounds__(256) void apply_fused_kernel(
         const float d1 = (float)((py - lp.c[1]) / Ly);
         const float d2 = (float)((pz - lp.c[2]) / Lz);
         const float* c = cl + (long long)l * kH;
-        // layer 1: this lane's 64 neurons 8s + 4h + i, as the B operands of layer 2
-        v4s hb[16];
+        // layer 1: this lane's 64 neurons 16s + 4h + i and 16s + 8 + 4h + i, as
+        // the B operands of layer 2
+        v8s hb[8];
 #pragma unroll
-        for (int s = 0; s < 16; s++) {
-            const float4 cc = *reinterpret_cast<const float4*>(c + 8 * s + 4 * h);
-            const float cv[4] = {cc.x, cc.y, cc.z, cc.w};
+        for (int s = 0; s < 8; s++) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const float4 w = sW1[8 * s + 4 * h + i];
-                hb[s][i] = (short)f2bf_d(siluf_(layer1_offsets(cv[i], w.x, w.y, w.z, d0, d1, d2)));
+            for (int half = 0; half < 2; half++) {
+                const int n = 16 * s + 8 * half + 4 * h;
+                const float4 cc = *reinterpret_cast<const float4*>(c + n);
+                const float cv[4] = {cc.x, cc.y, cc.z, cc.w};
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const float4 w = sW1[n + i];
+                    hb[s][4 * half + i] =
+                        (short)f2bf_d(siluf_(layer1_offsets(cv[i], w.x, w.y, w.z, d0, d1, d2)));
+                }
             }
         }
-        // layers 2 and 3: 128 x 128, 32-neuron tiles two at a time, 16 k-steps
+        // layers 2 and 3: 128 x 128, 32-neuron tiles two at a time, 8 k-steps
         hidden_layer(sW2, sb2, col, h, hb);
         hidden_layer(sW3, sb3, col, h, hb);
-        // layer 4: 32 outputs (3 used), 16 k-steps
+        // layer 4: 32 outputs (3 used), 8 k-steps
         v16f o4;
         for (int t = 0; t < 16; t++) o4[t] = 0.0f;
 #pragma unroll
-        for (int s = 0; s < 16; s++) {
-            const v4s wa = *reinterpret_cast<const v4s*>(sW4 + col * kLd + 8 * s + 4 * h);
-            o4 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(wa, hb[s], o4, 0, 0, 0);
-        }
+        for (int s = 0; s < 8; s++) o4 = mfma_k16(sW4 + col * kLd + 16 * s + 8 * h, hb[s], o4);
         const float sc = sl[l];
         ox = fmaf(sc, o4[0] + sb4[0], ox);
         oy = fmaf(sc, o4[1] + sb4[1], oy);
