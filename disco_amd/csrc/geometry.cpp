@@ -5,7 +5,9 @@
 // (:13-57), the loop constructors (:62-244), snapToMidPlane (:342-374) with
 // latticePlaneSpacing (Code/lattice.hpp:146-153), loop::polyline and
 // roundedPolygon2D (:478-580), generateLoopons / insideUV (:253-316) and
-// generateSurface (:426-435).
+// generateSurface (:426-435), and the Burgers-vector helpers of the prismatic
+// slabs: isLatticeTranslation, classifyBurgers and frankCopyShift
+// (Code/lattice.hpp:159-168,189-212,239-256).
 //
 // Rounding: the reference's host build (g++ -O3 -march=native) contracts
 // products into fused multiply-adds, so the same formula rounds differently
@@ -79,6 +81,20 @@ double plane_spacing(int lattice, const long h[3], double a0)
     else
         full = std::labs(h[0] + h[1] + h[2]) % 2 == 0;
     return full ? a0 / norm : a0 / (2.0 * norm);
+}
+
+// Is m (in units of a0/2) a translation of the lattice?  Integer components
+// within 1e-9; FCC: even sum; BCC: all of one parity (Code/lattice.hpp:159-168).
+bool lattice_translation(int lattice, const double m[3])
+{
+    long h[3];
+    for (int i = 0; i < 3; i++) {
+        h[i] = std::lround(m[i]);
+        if (std::fabs(m[i] - (double)h[i]) > 1e-9) return false;
+    }
+    if (lattice == DISCO_LATTICE_FCC_GEO) return std::labs(h[0] + h[1] + h[2]) % 2 == 0;
+    return (std::labs(h[0]) % 2 == std::labs(h[1]) % 2) &&
+           (std::labs(h[1]) % 2 == std::labs(h[2]) % 2);
 }
 
 // A loop as the reference's constructors leave it.
@@ -552,6 +568,96 @@ long long disco_geo_polylines(const disco_loop_shape* shapes, int nloops, double
             for (const V3& p : all[l]) { vx[o] = p.x; vy[o] = p.y; vz[o] = p.z; o++; }
         }
     return total;
+}
+
+int disco_geo_classify_burgers(int lattice, const double m[3], const double n[3])
+{
+    if (!m || !n) return DISCO_GEO_ERR_ARG;
+    if (lattice_translation(lattice, m)) return DISCO_BURGERS_PERFECT;
+    if (lattice != DISCO_LATTICE_FCC_GEO) return DISCO_BURGERS_OTHER;
+    long k[3];   // 3 m: integers for the 1/3 fractions of the partials
+    for (int i = 0; i < 3; i++) {
+        k[i] = std::lround(3.0 * m[i]);
+        if (std::fabs(std::fma(3.0, m[i], -(double)k[i])) > 1e-6) return DISCO_BURGERS_OTHER;
+    }
+    const long a = std::labs(k[0]), b = std::labs(k[1]), c = std::labs(k[2]);
+    if (a == 2 && b == 2 && c == 2) {   // 2/3<111>: a Frank partial when n is along m
+        const double x[3] = {std::fma(m[1], n[2], -(m[2] * n[1])),
+                             std::fma(m[2], n[0], -(m[0] * n[2])),
+                             std::fma(m[0], n[1], -(m[1] * n[0]))};
+        const double cn = std::sqrt(std::fma(x[2], x[2], std::fma(x[0], x[0], x[1] * x[1])));
+        const double nn = std::sqrt(std::fma(n[2], n[2], std::fma(n[0], n[0], n[1] * n[1])));
+        return cn < 1e-6 * nn ? DISCO_BURGERS_FRANK : DISCO_BURGERS_OTHER;
+    }
+    const long lo = std::min(a, std::min(b, c)), hi = std::max(a, std::max(b, c));
+    if (lo == 1 && hi == 2 && a + b + c == 4) {   // 1/3<112> in a {111} plane, m.n = 0
+        const bool n111 = std::fabs(n[0]) > 0.0 &&
+                          std::fabs(std::fabs(n[0]) - std::fabs(n[1])) < 1e-9 &&
+                          std::fabs(std::fabs(n[0]) - std::fabs(n[2])) < 1e-9;
+        const double mn = std::fma(m[2], n[2], std::fma(m[0], n[0], m[1] * n[1]));
+        return n111 && std::fabs(mn) < 1e-6 ? DISCO_BURGERS_SHOCKLEY : DISCO_BURGERS_OTHER;
+    }
+    return DISCO_BURGERS_OTHER;
+}
+
+void disco_geo_frank_copy_shift(const double v[3], const double nhat[3], double a0,
+                                double out[3])
+{
+    // the 24 vectors a0/6 <112>: the 2 sits on each axis in turn (cyclic),
+    // each of the 8 sign patterns; the first one in the plane whose sum with v
+    // is not a translation wins
+    for (int p = 0; p < 3; p++)
+        for (int sg = 0; sg < 8; sg++) {
+            double s[3];
+            for (int i = 0; i < 3; i++) {
+                const double sign = (sg >> i) & 1 ? -1.0 : 1.0;
+                const double base = i == 2 ? 2.0 : 1.0;
+                s[(p + i) % 3] = sign * base * a0 / 6.0;
+            }
+            const double sn = std::fma(s[2], nhat[2], std::fma(s[0], nhat[0], s[1] * nhat[1]));
+            if (std::fabs(sn) > 1e-9 * a0) continue;
+            const double w[3] = {2.0 * (v[0] + s[0]) / a0, 2.0 * (v[1] + s[1]) / a0,
+                                 2.0 * (v[2] + s[2]) / a0};
+            if (lattice_translation(DISCO_LATTICE_FCC_GEO, w)) continue;
+            for (int i = 0; i < 3; i++) out[i] = v[i] + s[i];
+            return;
+        }
+    for (int i = 0; i < 3; i++) out[i] = v[i];
+}
+
+void disco_geo_copy_shifts(const double b[3], int frank, const double n[3], const double* R,
+                           double a0, double shift[6])
+{
+    // rotateToCrystal / rotateToLab (Code/lattice.hpp:64-78): R^T v, R c
+    auto to_crystal = [R](const double* v, double* c) {
+        for (int i = 0; i < 3; i++)
+            c[i] = std::fma(R[6 + i], v[2], std::fma(R[i], v[0], R[3 + i] * v[1]));
+    };
+    auto to_lab = [R](const double* c, double* r) {
+        for (int i = 0; i < 3; i++)
+            r[i] = std::fma(R[3 * i + 2], c[2], std::fma(R[3 * i], c[0], R[3 * i + 1] * c[1]));
+    };
+    for (int sd = 0; sd < 2; sd++) {
+        const double sign = sd == 0 ? 1.0 : -1.0;
+        const double v[3] = {sign * b[0], sign * b[1], sign * b[2]};
+        double* out = shift + 3 * sd;
+        if (!frank) {
+            for (int i = 0; i < 3; i++) out[i] = v[i];
+            continue;
+        }
+        double nh[3], vc[3], w[3];
+        if (R) {
+            to_crystal(n, nh);
+            to_crystal(v, vc);
+        } else {
+            for (int i = 0; i < 3; i++) { nh[i] = n[i]; vc[i] = v[i]; }
+        }
+        disco_geo_frank_copy_shift(vc, nh, a0, w);
+        if (R)
+            to_lab(w, out);
+        else
+            for (int i = 0; i < 3; i++) out[i] = w[i];
+    }
 }
 
 }  // extern "C"

@@ -22,6 +22,7 @@
 // comparison sees the reference's values bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -211,6 +212,171 @@ __global__ __launch_bounds__(kSlabBlock) void slab_scatter_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Applying the slabs (Code/main.cpp:586-687, 733-742): order-keeping
+// compactions built from the same three steps -- per-tile counts of a
+// predicate, the scan of the counts, a scatter at tile offset + rank.  Pred is
+// a functor (bool)(k); Move (k, keep, pos) writes item k's data at pos.
+
+template <class Pred>
+__global__ __launch_bounds__(kSlabBlock) void keep_count_kernel(Pred pred, long long n,
+                                                                unsigned* __restrict__ tile_counts)
+{
+    __shared__ unsigned wcount[kSlabBlock / 64];
+    const long long t0 = (long long)blockIdx.x * kSlabTile;
+    unsigned count = 0;
+    for (int j = 0; j < kSlabTile; j += kSlabBlock) {
+        const long long k = t0 + j + threadIdx.x;
+        count += (unsigned)__popcll(__ballot(k < n && pred(k)));
+    }
+    if ((threadIdx.x & 63) == 0) wcount[threadIdx.x >> 6] = count;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = 0;
+        for (int w = 0; w < kSlabBlock / 64; w++) t += wcount[w];
+        tile_counts[blockIdx.x] = t;
+    }
+}
+
+template <class Pred, class Move>
+__global__ __launch_bounds__(kSlabBlock) void keep_scatter_kernel(
+    Pred pred, Move move, long long n, const unsigned long long* __restrict__ offsets)
+{
+    __shared__ unsigned wcount[kSlabBlock / 64];
+    const long long t0 = (long long)blockIdx.x * kSlabTile;
+    unsigned long long pos0 = offsets[blockIdx.x];
+    for (int j = 0; j < kSlabTile; j += kSlabBlock) {
+        const long long k = t0 + j + threadIdx.x;
+        const bool keep = k < n && pred(k);
+        unsigned rank;
+        const unsigned step = block_rank(keep, wcount, &rank);
+        if (k < n) move(k, keep, pos0 + rank);
+        pos0 += step;
+    }
+}
+
+// slab entries of one grain and twin region (Code/main.cpp:605-615)
+struct KeepRegion {
+    const int* atoms;
+    const int* grain;   // null: any grain
+    const int* twin;    // null: any region
+    int g, region;
+    long long natoms;   // entries naming no atom are dropped
+    __device__ bool operator()(long long q) const
+    {
+        const int a = atoms[q];
+        if (a < 0 || a >= natoms) return false;
+        return (!grain || grain[a] == g) && (!twin || twin[a] == region);
+    }
+};
+struct MoveSlab {
+    const int *atoms, *side;
+    int *atoms_o, *side_o;
+    __device__ void operator()(long long q, bool keep, unsigned long long pos) const
+    {
+        if (keep) { atoms_o[pos] = atoms[q]; side_o[pos] = side[q]; }
+    }
+};
+
+// atoms not removed (Code/main.cpp:664-676), with the new index of every atom
+struct KeepAtom {
+    const unsigned char* removed;
+    __device__ bool operator()(long long a) const { return removed[a] == 0; }
+};
+struct MoveAtom {
+    const double *x, *y, *z;
+    const int* label;
+    double *xo, *yo, *zo;
+    int* label_o;
+    int* new_index;
+    __device__ void operator()(long long a, bool keep, unsigned long long pos) const
+    {
+        new_index[a] = keep ? (int)pos : -1;
+        if (!keep) return;
+        xo[pos] = x[a];
+        yo[pos] = y[a];
+        zo[pos] = z[a];
+        if (label) label_o[pos] = label[a];
+    }
+};
+
+// copies whose source survives, renumbered (Code/main.cpp:679-682)
+struct KeepCopy {
+    const int* copy_atom;
+    const int* new_index;
+    long long n;   // copies naming no atom are dropped
+    __device__ bool operator()(long long k) const
+    {
+        const int a = copy_atom[k];
+        return a >= 0 && a < n && new_index[a] >= 0;
+    }
+};
+struct MoveCopy {
+    const int* copy_atom;
+    const int* new_index;
+    const double* shift;
+    int* atom_o;
+    double* shift_o;
+    __device__ void operator()(long long k, bool keep, unsigned long long pos) const
+    {
+        if (!keep) return;
+        atom_o[pos] = new_index[copy_atom[k]];
+        shift_o[3 * pos] = shift[3 * k];
+        shift_o[3 * pos + 1] = shift[3 * k + 1];
+        shift_o[3 * pos + 2] = shift[3 * k + 2];
+    }
+};
+
+// removed[atoms[q]] = 1 for a vacancy slab (Code/main.cpp:626-628)
+__global__ void mark_removed_kernel(long long natoms, long long count,
+                                    const int* __restrict__ atoms,
+                                    unsigned char* __restrict__ removed)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    const int a = atoms[q];
+    if (a >= 0 && a < natoms) removed[a] = 1;
+}
+
+// the copies of an interstitial slab: source and shift by side
+// (Code/main.cpp:652-655)
+__global__ void slab_copies_kernel(long long count, const int* __restrict__ atoms,
+                                   const int* __restrict__ side, double p0, double p1, double p2,
+                                   double m0, double m1, double m2, int* __restrict__ copy_atom,
+                                   double* __restrict__ copy_shift)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    const bool plus = side[q] > 0;
+    copy_atom[q] = atoms[q];
+    copy_shift[3 * q] = plus ? p0 : m0;
+    copy_shift[3 * q + 1] = plus ? p1 : m1;
+    copy_shift[3 * q + 2] = plus ? p2 : m2;
+}
+
+// entry n + k = copy k: the source's site and label, its displacement plus the
+// shift (Code/main.cpp:733-742)
+__global__ void append_copies_kernel(long long n, long long ncopies,
+                                     const int* __restrict__ copy_atom,
+                                     const double* __restrict__ shift, double* __restrict__ x,
+                                     double* __restrict__ y, double* __restrict__ z,
+                                     double* __restrict__ ux, double* __restrict__ uy,
+                                     double* __restrict__ uz, int* __restrict__ label)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncopies) return;
+    const int a = copy_atom[k];
+    if (a < 0 || a >= n) return;   // names no atom: entry n + k is not written
+    const long long o = n + k;
+    x[o] = x[a];
+    y[o] = y[a];
+    z[o] = z[a];
+    ux[o] = ux[a] + shift[3 * k];
+    uy[o] = uy[a] + shift[3 * k + 1];
+    uz[o] = uz[a] + shift[3 * k + 2];
+    if (label) label[o] = label[a];
+}
+
 int ensure(disco_ctx* ctx, void** p, size_t* cap, size_t bytes)
 {
     if (*cap >= bytes) return DISCO_OK;
@@ -221,6 +387,66 @@ int ensure(disco_ctx* ctx, void** p, size_t* cap, size_t bytes)
     *cap = bytes;
     return DISCO_OK;
 }
+
+// Scratch of a compaction over at most nmax items, plus `extra` bytes after it.
+struct Compaction {
+    unsigned long long* total;
+    unsigned* counts;
+    unsigned long long* offs;
+    char* extra;
+};
+
+int compaction_scratch(disco_ctx* ctx, long long nmax, size_t extra, Compaction* c)
+{
+    const long long nb = (nmax + kSlabTile - 1) / kSlabTile + 1;
+    const size_t o_counts = 256;
+    const size_t o_offs = (o_counts + (size_t)nb * 4 + 255) & ~(size_t)255;
+    const size_t o_extra = (o_offs + (size_t)nb * 8 + 255) & ~(size_t)255;
+    const int rc = ensure(ctx, &ctx->d_slab_scratch, &ctx->cap_slab_scratch, o_extra + extra);
+    if (rc) return rc;
+    char* sc = static_cast<char*>(ctx->d_slab_scratch);
+    c->total = reinterpret_cast<unsigned long long*>(sc);
+    c->counts = reinterpret_cast<unsigned*>(sc + o_counts);
+    c->offs = reinterpret_cast<unsigned long long*>(sc + o_offs);
+    c->extra = sc + o_extra;
+    return DISCO_OK;
+}
+
+// The pinned word the compaction totals come back through.
+int pinned_total(disco_ctx* ctx, unsigned long long** h)
+{
+    if (!ctx->h_slab_stage)
+        DISCO_HIP_TRY(ctx, hipHostMalloc(&ctx->h_slab_stage, sizeof(SlabLoop) + 256,
+                                         hipHostMallocDefault));
+    *h = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->h_slab_stage) +
+                                               ((sizeof(SlabLoop) + 63) & ~(size_t)63));
+    return DISCO_OK;
+}
+
+// Items k < n for which pred(k) holds, moved in order; the count is left in
+// c.total on the device (stream-ordered).
+template <class Pred, class Move>
+int launch_compaction(disco_ctx* ctx, long long n, Pred pred, Move move, const Compaction& c,
+                      hipStream_t s)
+{
+    const long long nb = (n + kSlabTile - 1) / kSlabTile;
+    if (nb > 0) {
+        hipLaunchKernelGGL((keep_count_kernel<Pred>), dim3((unsigned)nb), dim3(kSlabBlock), 0, s,
+                           pred, n, c.counts);
+        DISCO_HIP_TRY(ctx, hipGetLastError());
+    }
+    hipLaunchKernelGGL(slab_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, c.counts, c.offs, nb,
+                       c.total);
+    DISCO_HIP_TRY(ctx, hipGetLastError());
+    if (nb > 0) {
+        hipLaunchKernelGGL((keep_scatter_kernel<Pred, Move>), dim3((unsigned)nb),
+                           dim3(kSlabBlock), 0, s, pred, move, n, c.offs);
+        DISCO_HIP_TRY(ctx, hipGetLastError());
+    }
+    return DISCO_OK;
+}
+
+unsigned blocks256(long long n) { return (unsigned)((n + 255) / 256); }
 
 double dot3(const double* a, const double* b)
 {
@@ -341,6 +567,146 @@ int disco_hip_prismatic_slab(disco_ctx* ctx, const disco_loop_state* loop, doubl
     info->count = (long long)*h_total;
     if (info->count > cap)
         return fail(ctx, DISCO_ERR_TOO_LARGE, "slab larger than the output capacity");
+    return DISCO_OK;
+}
+
+int disco_hip_slab_filter(disco_ctx* ctx, long long natoms, const int* d_grain, int grain,
+                          const int* d_twin, int region, int* d_atoms, int* d_side,
+                          long long* count, void* stream)
+{
+    if (!ctx) return DISCO_ERR_ARG;
+    if (!count || *count < 0) return fail(ctx, DISCO_ERR_ARG, "null or negative count");
+    if (natoms < 0) return fail(ctx, DISCO_ERR_ARG, "negative atom count");
+    const long long n = *count;
+    if (n == 0 || (!d_grain && !d_twin)) return DISCO_OK;
+    if (!d_atoms || !d_side) return fail(ctx, DISCO_ERR_ARG, "null slab arrays");
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    unsigned long long* h_total;
+    int rc = pinned_total(ctx, &h_total);
+    if (rc) return rc;
+    Compaction c;
+    rc = compaction_scratch(ctx, n, (size_t)n * 8, &c);
+    if (rc) return rc;
+    int* tmp_atoms = reinterpret_cast<int*>(c.extra);
+    int* tmp_side = tmp_atoms + n;
+    hipStream_t s = (hipStream_t)stream;
+    rc = launch_compaction(ctx, n, KeepRegion{d_atoms, d_grain, d_twin, grain, region, natoms},
+                           MoveSlab{d_atoms, d_side, tmp_atoms, tmp_side}, c, s);
+    if (rc) return rc;
+    DISCO_HIP_TRY(ctx, hipMemcpyAsync(h_total, c.total, sizeof(*h_total), hipMemcpyDeviceToHost, s));
+    DISCO_HIP_TRY(ctx, hipStreamSynchronize(s));
+    const long long kept = (long long)*h_total;
+    if (kept > 0) {
+        DISCO_HIP_TRY(ctx, hipMemcpyAsync(d_atoms, tmp_atoms, (size_t)kept * 4,
+                                          hipMemcpyDeviceToDevice, s));
+        DISCO_HIP_TRY(ctx, hipMemcpyAsync(d_side, tmp_side, (size_t)kept * 4,
+                                          hipMemcpyDeviceToDevice, s));
+        DISCO_HIP_TRY(ctx, hipStreamSynchronize(s));
+    }
+    *count = kept;
+    return DISCO_OK;
+}
+
+int disco_hip_slab_mark_removed(disco_ctx* ctx, long long natoms, long long count,
+                                const int* d_atoms, unsigned char* d_removed, void* stream)
+{
+    if (!ctx) return DISCO_ERR_ARG;
+    if (count < 0 || natoms < 0) return fail(ctx, DISCO_ERR_ARG, "negative count");
+    if (count == 0) return DISCO_OK;
+    if (!d_atoms || !d_removed) return fail(ctx, DISCO_ERR_ARG, "null device array");
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(mark_removed_kernel, dim3(blocks256(count)), dim3(256), 0,
+                       (hipStream_t)stream, natoms, count, d_atoms, d_removed);
+    DISCO_HIP_TRY(ctx, hipGetLastError());
+    return DISCO_OK;
+}
+
+int disco_hip_slab_copies(disco_ctx* ctx, long long count, const int* d_atoms, const int* d_side,
+                          const double shift[6], int* d_copy_atom, double* d_copy_shift,
+                          void* stream)
+{
+    if (!ctx) return DISCO_ERR_ARG;
+    if (count < 0 || !shift) return fail(ctx, DISCO_ERR_ARG, "negative count or null shift");
+    if (count == 0) return DISCO_OK;
+    if (!d_atoms || !d_side || !d_copy_atom || !d_copy_shift)
+        return fail(ctx, DISCO_ERR_ARG, "null device array");
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(slab_copies_kernel, dim3(blocks256(count)), dim3(256), 0,
+                       (hipStream_t)stream, count, d_atoms, d_side, shift[0], shift[1], shift[2],
+                       shift[3], shift[4], shift[5], d_copy_atom, d_copy_shift);
+    DISCO_HIP_TRY(ctx, hipGetLastError());
+    return DISCO_OK;
+}
+
+int disco_hip_drop_atoms(disco_ctx* ctx, long long n, const unsigned char* d_removed,
+                         const double* d_x, const double* d_y, const double* d_z,
+                         const int* d_label, double* d_xo, double* d_yo, double* d_zo,
+                         int* d_label_o, long long ncopies, const int* d_copy_atom,
+                         const double* d_copy_shift, int* d_copy_atom_o, double* d_copy_shift_o,
+                         long long* n_kept, long long* ncopies_kept, void* stream)
+{
+    if (!ctx) return DISCO_ERR_ARG;
+    if (!n_kept || !ncopies_kept) return fail(ctx, DISCO_ERR_ARG, "null output count");
+    *n_kept = 0;
+    *ncopies_kept = 0;
+    if (n < 0 || ncopies < 0) return fail(ctx, DISCO_ERR_ARG, "negative size");
+    if (n > 0x7fffffffll) return fail(ctx, DISCO_ERR_TOO_LARGE, "atom indices are int");
+    if (n == 0) {
+        if (ncopies > 0) return fail(ctx, DISCO_ERR_ARG, "copies of atoms that do not exist");
+        return DISCO_OK;
+    }
+    if (!d_removed || !d_x || !d_y || !d_z || !d_xo || !d_yo || !d_zo ||
+        (d_label && !d_label_o))
+        return fail(ctx, DISCO_ERR_ARG, "null device array");
+    if (ncopies > 0 && (!d_copy_atom || !d_copy_shift || !d_copy_atom_o || !d_copy_shift_o))
+        return fail(ctx, DISCO_ERR_ARG, "null copy array");
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    unsigned long long* h_total;
+    int rc = pinned_total(ctx, &h_total);
+    if (rc) return rc;
+    Compaction c;
+    rc = compaction_scratch(ctx, std::max(n, ncopies), (size_t)n * 4, &c);
+    if (rc) return rc;
+    int* new_index = reinterpret_cast<int*>(c.extra);
+    hipStream_t s = (hipStream_t)stream;
+    rc = launch_compaction(ctx, n, KeepAtom{d_removed},
+                           MoveAtom{d_x, d_y, d_z, d_label, d_xo, d_yo, d_zo, d_label_o, new_index},
+                           c, s);
+    if (rc) return rc;
+    DISCO_HIP_TRY(ctx, hipMemcpyAsync(h_total, c.total, sizeof(*h_total), hipMemcpyDeviceToHost, s));
+    DISCO_HIP_TRY(ctx, hipStreamSynchronize(s));
+    *n_kept = (long long)*h_total;
+    if (ncopies > 0) {
+        rc = launch_compaction(ctx, ncopies, KeepCopy{d_copy_atom, new_index, n},
+                               MoveCopy{d_copy_atom, new_index, d_copy_shift, d_copy_atom_o,
+                                        d_copy_shift_o},
+                               c, s);
+        if (rc) return rc;
+        DISCO_HIP_TRY(ctx, hipMemcpyAsync(h_total, c.total, sizeof(*h_total),
+                                          hipMemcpyDeviceToHost, s));
+        DISCO_HIP_TRY(ctx, hipStreamSynchronize(s));
+        *ncopies_kept = (long long)*h_total;
+    }
+    return DISCO_OK;
+}
+
+int disco_hip_append_copies(disco_ctx* ctx, long long n, long long ncopies,
+                            const int* d_copy_atom, const double* d_copy_shift, double* d_x,
+                            double* d_y, double* d_z, double* d_ux, double* d_uy, double* d_uz,
+                            int* d_label, void* stream)
+{
+    if (!ctx) return DISCO_ERR_ARG;
+    if (n < 0 || ncopies < 0) return fail(ctx, DISCO_ERR_ARG, "negative size");
+    if (ncopies == 0) return DISCO_OK;
+    if (n == 0) return fail(ctx, DISCO_ERR_ARG, "copies of atoms that do not exist");
+    if (n + ncopies > 0x7fffffffll) return fail(ctx, DISCO_ERR_TOO_LARGE, "atom indices are int");
+    if (!d_copy_atom || !d_copy_shift || !d_x || !d_y || !d_z || !d_ux || !d_uy || !d_uz)
+        return fail(ctx, DISCO_ERR_ARG, "null device array");
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(append_copies_kernel, dim3(blocks256(ncopies)), dim3(256), 0,
+                       (hipStream_t)stream, n, ncopies, d_copy_atom, d_copy_shift, d_x, d_y, d_z,
+                       d_ux, d_uy, d_uz, d_label);
+    DISCO_HIP_TRY(ctx, hipGetLastError());
     return DISCO_OK;
 }
 

@@ -540,6 +540,9 @@ def _nlib():
                                           dp, _ct.c_longlong]
         L.disco_geo_polylines.restype = _ct.c_longlong
         L.disco_geo_loop_state.argtypes = [_ct.POINTER(_Shape), _ct.POINTER(_LoopState)]
+        L.disco_geo_classify_burgers.argtypes = [i, dp, dp]
+        L.disco_geo_frank_copy_shift.argtypes = [dp, dp, d, dp]
+        L.disco_geo_copy_shifts.argtypes = [dp, i, dp, vp, d, dp]
         _native = L
     return _native
 
@@ -646,3 +649,37 @@ def native_polylines(loops, eps: float = 0.01, nseg=None, subdivide=None):
     L.disco_geo_polylines(shapes, nl, float(eps), ns, sd, offp, _dptr(x), _dptr(y), _dptr(z),
                           total)
     return np.stack([x, y, z], 1), off
+
+
+BURGERS = {0: "perfect", 1: "Frank", 2: "Shockley", 3: "other"}
+
+
+def _v3(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(3))
+
+
+def native_classify_burgers(kind: str, m, n) -> int:
+    """classifyBurgers (Code/lattice.hpp:189-212) of b = (a0/2) m on the plane n,
+    crystal frame: 0 perfect, 1 Frank, 2 Shockley, 3 other."""
+    mm, nn = _v3(m), _v3(n)
+    return _nlib().disco_geo_classify_burgers(1 if kind == "FCC" else 0, _dptr(mm), _dptr(nn))
+
+
+def native_frank_copy_shift(v, nhat, a0: float):
+    """frankCopyShift (Code/lattice.hpp:239-256): v plus the in-plane a0/6<112>
+    that puts an inserted {111} layer on the third stacking position."""
+    vv, nn, out = _v3(v), _v3(nhat), np.zeros(3)
+    _nlib().disco_geo_frank_copy_shift(_dptr(vv), _dptr(nn), float(a0), _dptr(out))
+    return out
+
+
+def native_copy_shifts(b, frank: bool, n, R, a0: float):
+    """The copy shifts of an interstitial loop (Code/main.cpp:633-650): a (2, 3)
+    array, row 0 for the atoms on the +n side of the cut.  b = |b| m and n (unit)
+    in the lab frame; R the grain's 3x3 orientation (rows: crystal directions of
+    the box axes) or None."""
+    bb, nn, out = _v3(b), _v3(n), np.zeros(6)
+    Rm = None if R is None else np.ascontiguousarray(np.asarray(R, dtype=np.float64).reshape(9))
+    _nlib().disco_geo_copy_shifts(_dptr(bb), int(bool(frank)), _dptr(nn),
+                                  None if Rm is None else Rm.ctypes.data, float(a0), _dptr(out))
+    return out.reshape(2, 3)

@@ -18,6 +18,10 @@
  *   disco_geo_loop_state       the loop constructors    (Code/loop.cpp:13-244)
  *   disco_geo_polylines        the driver's per-loop LoopData assembly
  *                              (Code/main.cpp:689-711), for many loops at once
+ *   disco_geo_classify_burgers classifyBurgers     (Code/lattice.hpp:189-212)
+ *   disco_geo_frank_copy_shift frankCopyShift      (Code/lattice.hpp:239-256)
+ *   disco_geo_copy_shifts      the copy shifts of an interstitial loop
+ *                              (Code/main.cpp:633-650)
  */
 #ifndef DISCO_GEOMETRY_H
 #define DISCO_GEOMETRY_H
@@ -103,6 +107,31 @@ int disco_geo_loop_state(const disco_loop_shape *s, disco_loop_state *out);
 long long disco_geo_polylines(const disco_loop_shape *shapes, int nloops, double eps,
                               const int *nseg, const int *subdivide, long long *offsets,
                               double *vx, double *vy, double *vz, long long cap);
+
+/* Kind of the Burgers vector b = (a0/2) m of a loop of normal n (crystal
+ * frame, Miller indices): a lattice translation, an FCC Frank (2/3<111>, n
+ * along m) or Shockley (1/3<112> in a {111} plane) partial, or neither. */
+#define DISCO_BURGERS_PERFECT 0
+#define DISCO_BURGERS_FRANK 1
+#define DISCO_BURGERS_SHOCKLEY 2
+#define DISCO_BURGERS_OTHER 3
+int disco_geo_classify_burgers(int lattice, const double m[3], const double n[3]);
+/* Shift of the copy of an interstitial Frank-loop atom from its source (FCC,
+ * crystal frame): v = side * b plus the in-plane Shockley vector a0/6<112>
+ * (normal to nhat) for which the sum is not a lattice translation, so the
+ * inserted {111} layer takes the third stacking position.  out = v when no
+ * such vector exists (nhat not a <111> direction). */
+void disco_geo_frank_copy_shift(const double v[3], const double nhat[3], double a0,
+                                double out[3]);
+/* The two copy shifts of an interstitial loop (lab frame): shift[0..2] for the
+ * atoms on the +n side of the cut, shift[3..5] for the -n side.  v = +b / -b
+ * (b = |b| m, lab frame); for a Frank loop (frank != 0) frankCopyShift of v in
+ * the crystal frame, with the loop's unit normal n, brought back to the lab.
+ * R: the grain's orientation, rows = the crystal directions of the box axes
+ * (a crystal vector c is R c in the lab), row-major; NULL for an unrotated
+ * crystal. */
+void disco_geo_copy_shifts(const double b[3], int frank, const double n[3], const double *R,
+                           double a0, double shift[6]);
 
 #ifdef __cplusplus
 }

@@ -33,6 +33,17 @@ line_sum_ref32                golden barnettLoop<float> vectors (the float
 prismatic_slab, matter_type   the compiled reference's prismaticSlab on every
                               loop shape (rounded corners, axes, polygons);
                               golden slabs; brute force in exact arithmetic
+classify_burgers,             the compiled reference's classifyBurgers,
+frank_copy_shift, copy_shifts frankCopyShift and rotations on random and
+                              enumerated inputs; the reference unit test's
+                              Frank-copy checks (in-plane, |s| = a0/sqrt6, the
+                              third stacking position)
+prismatic_apply,              invariants of the driver's bookkeeping: kept
+append_copies                 atoms = the complement of the filtered vacancy
+                              slabs, in order; copies = the filtered
+                              interstitial entries in loop then slab order,
+                              renumbered, those of deleted atoms dropped;
+                              copies at their source's site with u + shift
 final_positions               exact rational arithmetic of the reference's
                               expression shapes under GCC contraction; wrap
                               closed forms; identity U leaves wrapped + L/2
@@ -95,6 +106,17 @@ def lib():
         L.oracle_prismatic_slab.argtypes = [ctypes.c_void_p, _d, ctypes.c_longlong, _dp, _dp,
                                             _dp, _d, _d, _d, _ip, _ip, ctypes.c_longlong, _dp]
         L.oracle_prismatic_slab.restype = ctypes.c_longlong
+        L.oracle_classify_burgers.argtypes = [_i, _dp, _dp]
+        L.oracle_classify_burgers.restype = _i
+        L.oracle_frank_copy_shift.argtypes = [_dp, _dp, _d, _dp]
+        L.oracle_copy_shifts.argtypes = [_dp, _i, _dp, ctypes.c_void_p, _d, _dp]
+        _llp = ctypes.POINTER(ctypes.c_longlong)
+        L.oracle_prismatic_apply.argtypes = [ctypes.c_longlong, _dp, _dp, _dp, ctypes.c_void_p,
+                                             ctypes.c_void_p, _i, _ip, _llp, _ip, _ip, _ip, _ip,
+                                             _dp, _ip, _dp, _llp]
+        L.oracle_prismatic_apply.restype = ctypes.c_longlong
+        L.oracle_append_copies.argtypes = [ctypes.c_longlong, ctypes.c_longlong, _ip, _dp, _dp,
+                                           _dp, _dp, _dp, _dp, _dp, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -130,6 +152,10 @@ def ref():
         R.ref_prismatic_slab.argtypes = [_i, _dp, _dp, _dp, _dp, _d, _d, _i, _dp, _dp, _dp, _d,
                                          _d, _d, _ip, _ip, _i, _dp]
         R.ref_prismatic_slab.restype = _i
+        R.ref_classify_burgers.argtypes = [_i, _dp, _dp]
+        R.ref_classify_burgers.restype = _i
+        R.ref_frank_copy_shift.argtypes = [_dp, _dp, _d, _dp]
+        R.ref_rotate.argtypes = [_dp, _i, _dp, _dp]
         _ref = R
     return _ref
 
@@ -364,6 +390,104 @@ def ref_prismatic_slab(shape, params, c, m, n, a0, spacing, x, y, z, box):
                                   _ptr(y), _ptr(z), float(box[0]), float(box[1]), float(box[2]),
                                   _ptr(atoms, _ip), _ptr(side, _ip), cap, _ptr(info))
     return atoms[:ns], side[:ns], info
+
+
+def classify_burgers(lattice, m, n) -> int:
+    """classifyBurgers restated (Code/lattice.hpp:189-212): 0 perfect, 1 Frank,
+    2 Shockley, 3 other; lattice "BCC" / "FCC"."""
+    return lib().oracle_classify_burgers(1 if lattice == "FCC" else 0, _ptr(_as3(m)),
+                                         _ptr(_as3(n)))
+
+
+def frank_copy_shift(v, nhat, a0):
+    """frankCopyShift restated (Code/lattice.hpp:239-256)."""
+    out = np.zeros(3)
+    lib().oracle_frank_copy_shift(_ptr(_as3(v)), _ptr(_as3(nhat)), float(a0), _ptr(out))
+    return out
+
+
+def copy_shifts(b, frank, n, R, a0):
+    """The two copy shifts of an interstitial loop (Code/main.cpp:633-650):
+    shape (2, 3), row 0 for side +1; R a 3x3 orientation (rows: crystal
+    directions of the box axes) or None."""
+    out = np.zeros(6)
+    Rm = None if R is None else np.ascontiguousarray(np.asarray(R, dtype=np.float64).reshape(9))
+    lib().oracle_copy_shifts(_ptr(_as3(b)), int(bool(frank)), _ptr(_as3(n)),
+                             None if Rm is None else Rm.ctypes.data, float(a0), _ptr(out))
+    return out.reshape(2, 3)
+
+
+def prismatic_apply(x, y, z, grain, twin, slabs):
+    """The driver's use of the slabs (Code/main.cpp:593-687), restated.
+
+    slabs: per loop, dict(type=0|1|2, atoms, side, grain=int (< 0 none),
+    region=int, shifts=(2, 3) array).  Returns (x, y, z, grain) compacted (grain
+    None when not given) and (copy_atom, copy_shift (k, 3))."""
+    x, y, z = (np.array(a, dtype=np.float64) for a in (x, y, z))
+    n = x.shape[0]
+    if grain is None and any(s.get("grain", -1) >= 0 for s in slabs):
+        raise ValueError("a loop names a grain but the atoms have no grain labels")
+    g = None if grain is None else np.array(grain, dtype=np.int32)
+    tw = None if twin is None else np.ascontiguousarray(twin, dtype=np.int32)
+    nl = len(slabs)
+    typ = np.array([s["type"] for s in slabs] or [0], dtype=np.int32)
+    off = np.zeros(nl + 1, dtype=np.int64)
+    for l, s in enumerate(slabs):
+        off[l + 1] = off[l] + len(s["atoms"])
+    atoms = np.ascontiguousarray(np.concatenate([np.asarray(s["atoms"], dtype=np.int32)
+                                                 for s in slabs] + [np.zeros(1, np.int32)]))
+    side = np.ascontiguousarray(np.concatenate([np.asarray(s["side"], dtype=np.int32)
+                                                for s in slabs] + [np.zeros(1, np.int32)]))
+    lg = np.array([s.get("grain", -1) for s in slabs] or [0], dtype=np.int32)
+    lr = np.array([s.get("region", 0) for s in slabs] or [0], dtype=np.int32)
+    sh = np.ascontiguousarray(np.concatenate([np.asarray(s.get("shifts", np.zeros((2, 3))),
+                                                         dtype=np.float64).reshape(6)
+                                              for s in slabs] + [np.zeros(6)]))
+    cap = max(int(off[-1]), 1)
+    ca, cs = np.zeros(cap, dtype=np.int32), np.zeros(3 * cap)
+    nc = ctypes.c_longlong(0)
+    k = lib().oracle_prismatic_apply(
+        n, _ptr(x), _ptr(y), _ptr(z), None if g is None else g.ctypes.data,
+        None if tw is None else tw.ctypes.data, nl, _ptr(typ, _ip),
+        off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _ptr(atoms, _ip), _ptr(side, _ip),
+        _ptr(lg, _ip), _ptr(lr, _ip), _ptr(sh), _ptr(ca, _ip), _ptr(cs), ctypes.byref(nc))
+    if k < 0:
+        raise MemoryError("oracle_prismatic_apply")
+    c = nc.value
+    return ((x[:k], y[:k], z[:k], None if g is None else g[:k]),
+            (ca[:c].copy(), cs[:3 * c].reshape(c, 3).copy()))
+
+
+def append_copies(x, y, z, ux, uy, uz, grain, copy_atom, copy_shift):
+    """The copies appended after the sum (Code/main.cpp:733-742), restated:
+    new arrays of n + ncopies entries."""
+    n, c = len(x), len(copy_atom)
+    out = [np.concatenate([np.asarray(a, dtype=np.float64), np.zeros(c)])
+           for a in (x, y, z, ux, uy, uz)]
+    g = None if grain is None else np.concatenate([np.asarray(grain, dtype=np.int32),
+                                                   np.zeros(c, dtype=np.int32)])
+    ca = np.ascontiguousarray(copy_atom, dtype=np.int32)
+    cs = np.ascontiguousarray(np.asarray(copy_shift, dtype=np.float64).reshape(-1))
+    lib().oracle_append_copies(n, c, _ptr(ca, _ip), _ptr(cs), *(_ptr(a) for a in out),
+                               None if g is None else g.ctypes.data)
+    return out, g
+
+
+def ref_classify_burgers(lattice, m, n) -> int:
+    return ref().ref_classify_burgers(1 if lattice == "FCC" else 0, _ptr(_as3(m)), _ptr(_as3(n)))
+
+
+def ref_frank_copy_shift(v, nhat, a0):
+    out = np.zeros(3)
+    ref().ref_frank_copy_shift(_ptr(_as3(v)), _ptr(_as3(nhat)), float(a0), _ptr(out))
+    return out
+
+
+def ref_rotate(R, to_lab, v):
+    out = np.zeros(3)
+    Rm = np.ascontiguousarray(np.asarray(R, dtype=np.float64).reshape(9))
+    ref().ref_rotate(_ptr(Rm), int(bool(to_lab)), _ptr(_as3(v)), _ptr(out))
+    return out
 
 
 # ---------------------------------------------------------------- output stage

@@ -15,6 +15,9 @@
 //                            (Code/loop.cpp:64-244,253-316,426-435,536-608,342-374)
 //   ref_lattice           -> generateLattice       (Code/lattice.hpp:35-51)
 //   ref_prismatic_slab    -> prismaticSlab         (Code/prismatic.cpp:30-74)
+//   ref_classify_burgers  -> classifyBurgers       (Code/lattice.hpp:189-212)
+//   ref_frank_copy_shift  -> frankCopyShift        (Code/lattice.hpp:239-256)
+//   ref_rotate            -> rotateToLab / rotateToCrystal (Code/lattice.hpp:64-78)
 #include <cstring>
 #include <iostream>
 #include <sstream>
@@ -247,6 +250,34 @@ int ref_prismatic_slab(int shape, const double* p, const double* c, const double
     info[3] = r.spacing;
     for (int i = 0; i < 3; i++) info[4 + i] = r.b[i];
     return ns;
+}
+
+// classifyBurgers (Code/lattice.hpp:189-212): 0 perfect, 1 Frank, 2 Shockley, 3 other
+int ref_classify_burgers(int lattice, const double* m, const double* n)
+{
+    switch (classifyBurgers(lattice ? Lattice::FCC : Lattice::BCC, v3(m), v3(n))) {
+        case BurgersClass::PERFECT: return 0;
+        case BurgersClass::FRANK: return 1;
+        case BurgersClass::SHOCKLEY: return 2;
+        default: return 3;
+    }
+}
+
+// frankCopyShift (Code/lattice.hpp:239-256)
+void ref_frank_copy_shift(const double* v, const double* nhat, double a0, double* out)
+{
+    frankCopyShift(v, nhat, a0, out);
+}
+
+// rotateToLab (to_lab != 0) or rotateToCrystal of v with the row-major R
+void ref_rotate(const double* R, int to_lab, const double* v, double* out)
+{
+    Orientation o;
+    o.active = true;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) o.R[i][j] = R[3 * i + j];
+    const vector<double> r = to_lab ? rotateToLab(o, v3(v)) : rotateToCrystal(o, v3(v));
+    for (int i = 0; i < 3; i++) out[i] = r[i];
 }
 
 }  // extern "C"
