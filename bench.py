@@ -31,7 +31,8 @@ METRIC = ("atom·segment·image evals/s and ms per 4M-atom sample at "
 # Algorithmic FLOPs per atom x segment x image of the line integral: 93
 # add/sub/mul + 4 div + 1 sqrt + 1 log + 1 atan2, each counted as 1.
 FLOP_PER_EVAL = 100.0
-FP32_VECTOR_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
+FP32_VECTOR_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak (packed FP32 FMA)
+FP32_UNPACKED_PEAK_TFLOPS = 78.6  # the same without packed FP32 (one FMA per lane per clock)
 # HBM bytes per atom of the far pass, measured with rocprofv3 PMC counters
 # (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, each scaled by a plain copy of
 # known size, `--workload copy`): 193.9e6 B per dispatch over 4,000,000 atoms.
@@ -173,6 +174,8 @@ def main():
     roofline = {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
                 "traffic": FAR_PASS_HBM_BYTES_PER_ATOM * s.n, "traffic_unit": "bytes",
+                "peak_unpacked": FP32_UNPACKED_PEAK_TFLOPS,
+                "frac_unpacked": achieved / FP32_UNPACKED_PEAK_TFLOPS,
                 "kernel": "line_kernel<PASS_FAR, LOOP_GLIDE>", "kernel_ms": far,
                 "near_kernel_ms": float(np.mean(near_ms)),
                 "flop_per_eval": FLOP_PER_EVAL, "evals_per_launch": evals}
