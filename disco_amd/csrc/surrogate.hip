@@ -460,7 +460,15 @@ __device__ __forceinline__ void hidden_layer(const bf16* __restrict__ W,
 // t%4) is exactly the B operand of the next layer's two K16 steps, so
 // activations never leave registers.  Layer 1 (three offset features; the
 // loop's features are in cl) runs on the VALU in FP32.
-__global__ __launch_bounds__(256) void apply_fused_kernel(
+//
+// Occupancy: the weights take 81.5 KB of LDS, so two workgroups fit a CU; with
+// 8 waves per workgroup and at most 128 registers per lane (4 waves per SIMD,
+// the launch bound) that is 16 waves per CU, twice the 4-wave workgroup's
+// count -- the MFMA chains of one wave and the SiLUs of another overlap.
+constexpr int kApplyWaves = 8;                  // waves per workgroup, 32 atoms each
+constexpr int kApplyThreads = 64 * kApplyWaves;
+constexpr int kApplyAtoms = 32 * kApplyWaves;
+__global__ __launch_bounds__(kApplyThreads, 4) void apply_fused_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     long long a0, int na, const disco_surrogate_loop* __restrict__ loops, int nl, double Lx,
     double Ly, double Lz, const float* __restrict__ P, const bf16* __restrict__ Wb,
@@ -470,14 +478,14 @@ __global__ __launch_bounds__(256) void apply_fused_kernel(
     __shared__ bf16 sW2[kH * kLd], sW3[kH * kLd], sW4[kOut * kLd];
     __shared__ float4 sW1[kH];
     __shared__ float sb2[kH], sb3[kH], sb4[kOut];
-    for (int i = threadIdx.x; i < kH * kH; i += 256) {
+    for (int i = threadIdx.x; i < kH * kH; i += kApplyThreads) {
         const int r = i / kH, c = i % kH;
         const int p = r * kLd + k16_col(c);
         sW2[p] = Wb[i];
         sW3[p] = Wb[kH * kH + i];
         if (r < kOut) sW4[p] = Wb[2 * kH * kH + i];
     }
-    for (int i = threadIdx.x; i < kH; i += 256) {
+    for (int i = threadIdx.x; i < kH; i += kApplyThreads) {
         const float* w = P + oW1 + i * kIn;
         sW1[i] = make_float4(w[0], w[1], w[2], 0.0f);
         sb2[i] = P[oB2 + i];
@@ -487,7 +495,7 @@ __global__ __launch_bounds__(256) void apply_fused_kernel(
     __syncthreads();
 
     const int lane = threadIdx.x & 63, col = lane & 31, h = lane >> 5;
-    const int ia = blockIdx.x * 128 + (threadIdx.x >> 6) * 32 + col;   // atom of this column
+    const int ia = blockIdx.x * kApplyAtoms + (threadIdx.x >> 6) * 32 + col;   // atom of this column
     const bool valid = ia < na;
     const long long a = a0 + (valid ? ia : na - 1);
     const double px = x[a], py = y[a], pz = z[a];
@@ -930,7 +938,8 @@ int disco_hip_surrogate_apply(disco_surrogate* s, int natoms, const double* x, c
                        0, st, s->P, dl, nloops, L[0], L[1], L[2], s->gain, cl, sl);
     for (long long a0 = 0; a0 < natoms; a0 += apc) {
         const int na = (int)std::min<long long>(apc, natoms - a0);
-        hipLaunchKernelGGL(apply_fused_kernel, dim3(blocks(na, 128)), dim3(256), 0, st, x, y, z,
+        hipLaunchKernelGGL(apply_fused_kernel, dim3(blocks(na, kApplyAtoms)), dim3(kApplyThreads),
+                           0, st, x, y, z,
                            a0, na, dl, nloops, L[0], L[1], L[2], s->P, s->Wb, cl, sl, ux, uy,
                            uz);
     }
