@@ -532,11 +532,21 @@ template <class M>
 DISCO_HD double two_atanh_vote_d(double z)
 {
     const double y = z * z;
-    double p = 1.0 / 27;
-    p = p * y + 1.0 / 25; p = p * y + 1.0 / 23; p = p * y + 1.0 / 21;
-    p = p * y + 1.0 / 19; p = p * y + 1.0 / 17; p = p * y + 1.0 / 15;
+    // z < 1/16: the series to z^15 (the next term is below 2^-60 of the sum);
+    // the one to z^27 for z < 1/4 only when a lane of the wave needs it, so
+    // each lane's result depends on its own z alone
+    double p = 1.0 / 15;
     p = p * y + 1.0 / 13; p = p * y + 1.0 / 11; p = p * y + 1.0 / 9;
     p = p * y + 1.0 / 7;  p = p * y + 1.0 / 5;  p = p * y + 1.0 / 3;
+    const bool mid = !(z < 0.0625);
+    if (__builtin_expect(M::any(mid), 0)) {
+        double q = 1.0 / 27;
+        q = q * y + 1.0 / 25; q = q * y + 1.0 / 23; q = q * y + 1.0 / 21;
+        q = q * y + 1.0 / 19; q = q * y + 1.0 / 17; q = q * y + 1.0 / 15;
+        q = q * y + 1.0 / 13; q = q * y + 1.0 / 11; q = q * y + 1.0 / 9;
+        q = q * y + 1.0 / 7;  q = q * y + 1.0 / 5;  q = q * y + 1.0 / 3;
+        p = mid ? q : p;
+    }
     double r = 2.0 * z + (2.0 * z * y) * p;
     const bool big = !(z < 0.25);
     if (__builtin_expect(M::any(big), 0)) {                    // a branch, not selects
