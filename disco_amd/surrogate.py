@@ -21,7 +21,7 @@ is the far shell.  Its surrogate is
 The near shell (+-1) is always the exact line integral.  ``far_shell`` checks
 the surrogate against the exact far shell on a validation subset of atoms and
 falls back to the exact sum when the error exceeds the tolerance -- the plain
-sum is both the fallback and the judge.
+sum is both the fallback and the measure of its error.
 """
 from __future__ import annotations
 
@@ -444,7 +444,7 @@ def far_shell(ctx, sur: Surrogate, x, y, z, u, loops, infos, box, img, tol=1e-5,
     rng = np.random.default_rng(seed)
     idx = torch.from_numpy(np.sort(rng.choice(n, min(n_valid, n), replace=False))).to(x.device)
     xv, yv, zv = x[idx].contiguous(), y[idx].contiguous(), z[idx].contiguous()
-    # judge: exact far shell on the validation atoms
+    # the reference for the error: exact far shell on the validation atoms
     ex = exact_far_shell_torch(ctx, xv, yv, zv, loops, box, img)
     # the base as the full run computes it (on the grid of the full atom set)
     dv = [torch.zeros_like(xv) for _ in range(3)]
