@@ -133,21 +133,30 @@ _DB = np.array([[1 / 3, 1 / 3, 1 / 3], [_A1, _B1, _B1], [_B1, _A1, _B1], [_B1, _
                 [_A2, _B2, _B2], [_B2, _A2, _B2], [_B2, _B2, _A2]])
 
 
+def _cross_rows(a, b):
+    """Row-wise cross products of (k, 3) arrays (np.cross without its per-call
+    axis handling, which dominates for the few rows of a loop)."""
+    return np.stack([a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1],
+                     a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2],
+                     a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]], 1)
+
+
 def area_moments(lp: B.LoopData):
     """Area A, normal n and in-plane moments I2 = int r^2 dA, I4 = int r^4 dA of
     the flat cut about its centre (fan triangles, exact degree-5 rule)."""
     c = np.array([lp.cx, lp.cy, lp.cz])
     V = np.stack([lp.vx, lp.vy, lp.vz], 1) - c
-    N = np.cross(np.roll(V, 1, axis=0), V).sum(0)
+    # every fan triangle (centre, V_j, V_j+1) at once: its area and the rule's
+    # points (the centre is the origin here)
+    P0, P1 = V, np.roll(V, -1, axis=0)
+    X = _cross_rows(P0, P1)
+    N = X.sum(0)
     A = 0.5 * float(np.linalg.norm(N))
-    I2 = I4 = 0.0
-    for j in range(len(V)):
-        P0, P1 = V[j], V[(j + 1) % len(V)]
-        ar = 0.5 * float(np.linalg.norm(np.cross(P0, P1)))
-        pts = _DB[:, 1:2] * P0 + _DB[:, 2:3] * P1
-        r2 = (pts ** 2).sum(1)
-        I2 += ar * float((_DW * r2).sum())
-        I4 += ar * float((_DW * r2 * r2).sum())
+    ar = 0.5 * np.sqrt((X * X).sum(1))                                      # (nv,)
+    pts = _DB[:, 1, None, None] * P0[None] + _DB[:, 2, None, None] * P1[None]   # (7, nv, 3)
+    r2 = (pts ** 2).sum(2)                                                  # (7, nv)
+    I2 = float(ar @ (_DW @ r2))
+    I4 = float(ar @ (_DW @ (r2 * r2)))
     return A, N / np.linalg.norm(N), I2, I4
 
 
@@ -156,32 +165,65 @@ def rule7_loopons(lp: B.LoopData):
     ring of radius rho, with w0 + 6 w1 = A, 6 w1 rho^2 = I2, 6 w1 rho^4 = I4
     (for a disc: rho^2 = 2R^2/3, w0 = A/4, w1 = A/8 -- Radon's degree-5 rule).
     Returns two loopon LoopData (centre, ring) sharing b, m, n."""
-    A, n, I2, I4 = area_moments(lp)
-    c = np.array([lp.cx, lp.cy, lp.cz])
-    rho = np.sqrt(I4 / I2)
-    w1 = I2 * I2 / (6.0 * I4)
-    w0 = A - 6.0 * w1
-    V0 = np.array([lp.vx[0], lp.vy[0], lp.vz[0]]) - c
-    e1 = V0 - np.dot(V0, n) * n
-    e1 /= np.linalg.norm(e1)
-    e2 = np.cross(n, e1)
+    return rule7_loopons_many([lp])
+
+
+def rule7_loopons_many(loops):
+    """rule7_loopons of many loops, computed together for the loops of each
+    vertex count: [centre, ring] per loop, in the loops' order."""
+    out = [None] * (2 * len(loops))
+    by_nv = {}
+    for i, lp in enumerate(loops):
+        by_nv.setdefault(len(lp.vx), []).append(i)
     th = 2.0 * np.pi * np.arange(6) / 6.0
-    ring = c + rho * (np.cos(th)[:, None] * e1 + np.sin(th)[:, None] * e2)
-    b = np.array(lp.burgers())
-    bm = float(np.linalg.norm(b))
-    m = b / bm
-    centre = B.LoopData(phys=(bm, bm, bm, *m, *n, w0), line=False, x=c[:1].copy(),
-                        y=c[1:2].copy(), z=c[2:3].copy())
-    ringd = B.LoopData(phys=(bm, bm, bm, *m, *n, w1), line=False, x=ring[:, 0].copy(),
-                       y=ring[:, 1].copy(), z=ring[:, 2].copy())
-    return [centre, ringd]
+    cs, sn = np.cos(th), np.sin(th)
+    for nv, idx in by_nv.items():
+        C = np.array([[loops[i].cx, loops[i].cy, loops[i].cz] for i in idx])        # (L, 3)
+        V = np.stack([np.stack([loops[i].vx, loops[i].vy, loops[i].vz], 1)
+                      for i in idx]) - C[:, None, :]                                  # (L, nv, 3)
+        P1 = np.roll(V, -1, axis=1)
+        X = np.stack([V[..., 1] * P1[..., 2] - V[..., 2] * P1[..., 1],
+                      V[..., 2] * P1[..., 0] - V[..., 0] * P1[..., 2],
+                      V[..., 0] * P1[..., 1] - V[..., 1] * P1[..., 0]], -1)           # (L, nv, 3)
+        N = X.sum(1)
+        nN = np.sqrt((N * N).sum(1))
+        A = 0.5 * nN
+        n = N / nN[:, None]
+        ar = 0.5 * np.sqrt((X * X).sum(2))                                            # (L, nv)
+        pts = (_DB[None, :, 1, None, None] * V[:, None] +
+               _DB[None, :, 2, None, None] * P1[:, None])                             # (L, 7, nv, 3)
+        r2 = (pts * pts).sum(3)                                                       # (L, 7, nv)
+        I2 = np.einsum("lj,q,lqj->l", ar, _DW, r2)
+        I4 = np.einsum("lj,q,lqj->l", ar, _DW, r2 * r2)
+        rho = np.sqrt(I4 / I2)
+        w1 = I2 * I2 / (6.0 * I4)
+        w0 = A - 6.0 * w1
+        V0 = V[:, 0]
+        e1 = V0 - (V0 * n).sum(1)[:, None] * n
+        e1 /= np.sqrt((e1 * e1).sum(1))[:, None]
+        e2 = np.stack([n[:, 1] * e1[:, 2] - n[:, 2] * e1[:, 1],
+                       n[:, 2] * e1[:, 0] - n[:, 0] * e1[:, 2],
+                       n[:, 0] * e1[:, 1] - n[:, 1] * e1[:, 0]], 1)
+        ring = C[:, None] + rho[:, None, None] * (cs[None, :, None] * e1[:, None] +
+                                                 sn[None, :, None] * e2[:, None])     # (L, 6, 3)
+        for k, i in enumerate(idx):
+            b = np.array(loops[i].burgers())
+            bm = float(np.linalg.norm(b))
+            m = b / bm
+            nk = n[k]
+            out[2 * i] = B.LoopData(phys=(bm, bm, bm, *m, *nk, float(w0[k])), line=False,
+                                    x=C[k, :1].copy(), y=C[k, 1:2].copy(), z=C[k, 2:3].copy())
+            out[2 * i + 1] = B.LoopData(phys=(bm, bm, bm, *m, *nk, float(w1[k])), line=False,
+                                        x=ring[k, :, 0].copy(), y=ring[k, :, 1].copy(),
+                                        z=ring[k, :, 2].copy())
+    return out
 
 
 def base_loopons(loops, kind: str):
     if kind == "dipole":
         return [dipole_loopon(l) for l in loops]
     if kind == "rule7":
-        return [q for l in loops for q in rule7_loopons(l)]
+        return rule7_loopons_many(loops)
     raise ValueError(f"unknown baseline {kind!r}")
 
 
