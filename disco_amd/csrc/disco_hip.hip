@@ -373,9 +373,11 @@ __device__ __forceinline__ void atom_loopon(double px, double py, double pz,
 
 // ------------------------------------------------------------------ one-sample kernels
 // FP64 line sum (PASS_FP64) or the FP32C far pass over loops of kind KIND
-// (PASS_FAR): write u (or add).
+// (PASS_FAR): write u (or add).  At least 4 waves per SIMD: the FP64
+// instantiations fit 128 registers (from 132, two values spilled outside the
+// segment walk) and run ~2 % faster; the FP32 ones use 78 and are unaffected.
 template <int PASS, int KIND>
-__global__ __launch_bounds__(kBlock) void line_kernel(
+__global__ __launch_bounds__(kBlock, 4) void line_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
     const LineLoop* __restrict__ loops, int nloops, const SegF* __restrict__ segf,
