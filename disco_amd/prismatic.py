@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import ctypes
 
-from .backend import DiscoError, lib
+from .backend import lib
 
 MATTER = {0: "glide", 1: "vacancy", 2: "interstitial"}
 

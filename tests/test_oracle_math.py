@@ -9,7 +9,6 @@ import pytest
 
 import oracle
 from disco_amd import geometry as G
-from disco_amd.backend import LoopData
 
 NU = 0.35
 K1 = 1.0 / (8.0 * math.pi * (1.0 - NU))
