@@ -86,13 +86,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--workload", default="c2",
-                    choices=["c2", "c3", "c4", "c5", "copy", "lattice", "writers", "prismatic"],
+                    choices=["c2", "c3", "c4", "c5", "copy", "lattice", "writers", "prismatic",
+                             "apply"],
                     help="c2 (default): the 4M-atom line-integral sample; c3: loopons; "
                          "c4: 256 batched samples; c5: the far-shell surrogate; copy: a plain "
                          "256 MiB device copy (x3), the size reference for HBM byte counters; "
                          "lattice: the 4M-atom lattice generated on the device; writers: the "
                          "4M-atom sample's final positions and .xyz / .lmp files; prismatic: "
-                         "slab selection of 20 prismatic loops over the 4M atoms")
+                         "slab selection of 20 prismatic loops over the 4M atoms; apply: the "
+                         "far-shell MLP over the 4M atoms and 200 loops")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): atoms shard "

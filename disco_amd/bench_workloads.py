@@ -609,10 +609,57 @@ def run_prismatic(args):
     ctx.close()
 
 
+def run_apply(args):
+    """The far-shell MLP alone: apply_fused_kernel over the 4M-atom sample (FCC
+    100^3) for 200 loops -- 8e8 (atom, loop) pairs x 69,888 MFMA FLOP (layers
+    2-4 on v_mfma_f32_32x32x16_bf16), with the weights of a fresh network (the
+    throughput does not depend on their values).  Its parity with the forward
+    GEMMs and the FP64 restatement is tested; no host baseline is timed here (the
+    workload that uses it, c5, times the plain sum beside it)."""
+    import torch
+    from . import backend as B
+    from . import surrogate as S
+    ctx = B.Context(0)
+    x, y, z = B.lattice_torch(ctx, "FCC", 100, 100, 100, 3.615)
+    box = [361.5] * 3
+    infos = [i for _, i in S.random_fcc_loops(np.random.default_rng(20245), 200)]
+    sur = S.Surrogate(ctx, seed=3)
+    u = [torch.zeros_like(x) for _ in range(3)]
+    for _ in range(max(1, args.warmup)):
+        sur.apply_torch(x, y, z, *u, infos, box)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ms = []
+    for _ in range(args.steps):
+        ev[0].record()
+        sur.apply_torch(x, y, z, *u, infos, box)
+        ev[1].record()
+        torch.cuda.synchronize()
+        ms.append(ev[0].elapsed_time(ev[1]))
+    t = float(np.mean(ms))
+    pairs = x.numel() * len(infos)
+    tflops = pairs * 69888.0 / (t * 1e-3) / 1e12
+    print(f"[apply] {pairs:.3e} (atom, loop) pairs in {t:.1f} ms: {tflops:.1f} TFLOP/s on MFMA",
+          flush=True)
+    _emit({"metric": "(atom, loop) pairs/s through the far-shell MLP", "value": pairs / (t * 1e-3),
+           "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": t, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16 (fp32 accumulate; layer 1 fp32)", "data": "synthetic",
+           "config": {"workload": "FCC a0=3.615 100^3 (4,000,000 atoms) x 200 loops, MLP "
+                                  "14 -> 128 -> 128 -> 128 -> 3 (random weights)"},
+           "roofline": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                        "frac": tflops / 2500.0, "traffic": None,
+                        "kernel": "apply_fused_kernel (HIP events around the call)",
+                        "kernel_ms": t, "flop_per_pair": 69888},
+           "cpu_baseline": None})
+    sur.free()
+    ctx.close()
+
+
 def run(args):
     if not hasattr(args, "train_loops"):
         args.train_loops = 2000
     if not hasattr(args, "epochs"):
         args.epochs = 30
     {"c3": run_c3, "c4": run_c4, "c5": run_c5, "lattice": run_lattice,
-     "writers": run_writers, "prismatic": run_prismatic}[args.workload](args)
+     "writers": run_writers, "prismatic": run_prismatic, "apply": run_apply}[args.workload](args)
