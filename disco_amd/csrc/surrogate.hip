@@ -6,7 +6,7 @@
 //     area at its centre (the Kroupa term, Code/kroupa.hpp:52-73), computed
 //     exactly by the loopon kernels with an image shell;
 //   * a residual, predicted by a small MLP:
-//       features (14, padded to 16) -> 128 -> 128 -> 128 -> 3, SiLU,
+//       features (14, padded to 16) -> 128 -> 128 -> 128 -> 3, hard swish,
 //     scaled per loop by |b| (R / L)^4 (the residual of a flat loop about its
 //     dipole scales with the fourth power of its size over the distance).
 // Layer 1 runs in FP32 on the VALU (the atom offset as a BF16 input would be
@@ -70,12 +70,20 @@ __host__ __device__ inline float bf2f(bf16 h)
     std::memcpy(&f, &u, 4);
     return f;
 }
-// logistic function: hardware exp and reciprocal (one rounding each)
-__device__ inline float sigmoidf_(float z)
+// The activation: hard swish, z clamp(z/6 + 1/2, 0, 1) -- an FMA with its clamp
+// (one v_med3) and a multiply, no transcendental (the logistic of SiLU costs an
+// exp and a reciprocal, which bounded the fused apply).  Derivative: 0 below
+// -3, 1 above 3, z/3 + 1/2 between.
+__device__ inline float hswish_gate(float z)
 {
-    return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+    return __builtin_amdgcn_fmed3f(fmaf(z, 1.0f / 6.0f, 0.5f), 0.0f, 1.0f);
 }
-__device__ inline float siluf_(float z) { return z * sigmoidf_(z); }
+__device__ inline float act_(float z) { return z * hswish_gate(z); }
+__device__ inline float act_grad(float z)
+{
+    const float g = fmaf(z, 1.0f / 6.0f, 0.5f);
+    return g <= 0.0f ? 0.0f : (g >= 1.0f ? 1.0f : fmaf(z, 1.0f / 3.0f, 0.5f));
+}
 
 // Layer-1 pre-activation of neuron j: the loop part first, in feature order
 // 3..15, then the three offset features -- the order the fused apply uses with
@@ -121,7 +129,7 @@ __host__ __device__ inline double pair_scale(const disco_surrogate_loop& lp, con
 }
 
 // ------------------------------------------------------------------ kernels
-// Layer 1 in FP32: Z = X W1^T + b1, A = bf16(silu(Z)); one thread per output.
+// Layer 1 in FP32: Z = X W1^T + b1, A = bf16(act(Z)); one thread per output.
 __global__ void layer1_kernel(const float* __restrict__ X, const float* __restrict__ P,
                               float* __restrict__ Z, bf16* __restrict__ A, int M)
 {
@@ -133,7 +141,7 @@ __global__ void layer1_kernel(const float* __restrict__ X, const float* __restri
     const float z = layer1_offsets(layer1_loop_part(w, P[oB1 + j], x), w[0], w[1], w[2], x[0],
                                    x[1], x[2]);
     if (Z) Z[t] = z;
-    A[t] = f2bf_d(siluf_(z));
+    A[t] = f2bf_d(act_(z));
 }
 
 // C (M x N) = A (M x K) . B (K x N) on MFMA, BF16 operands, FP32 accumulate.
@@ -142,7 +150,7 @@ __global__ void layer1_kernel(const float* __restrict__ X, const float* __restri
 // apply (so the forward and the apply round the same sums alike); grid (ceil(N/32), ceil(M/32)); out-of-range loads read zero
 // and out-of-range outputs are not stored.
 // EPI 0: C = acc (+ bias)                (fp32 out)
-// EPI 1: Z = acc + bias, A = bf16(silu)  (fp32 Z may be null, bf16 Ab)
+// EPI 1: Z = acc + bias, A = bf16(act)   (fp32 Z may be null, bf16 Ab)
 template <int EPI>
 __global__ __launch_bounds__(64) void gemm_bf16_kernel(
     const bf16* __restrict__ A, long long sam, long long sak, const bf16* __restrict__ B,
@@ -176,7 +184,7 @@ __global__ __launch_bounds__(64) void gemm_bf16_kernel(
             C[ic * ldc + jc] = z;
         } else {
             if (C) C[ic * ldc + jc] = z;
-            Ab[ic * ldab + jc] = f2bf_d(siluf_(z));
+            Ab[ic * ldab + jc] = f2bf_d(act_(z));
         }
     }
 }
@@ -203,14 +211,13 @@ __global__ void loss_kernel(const float* __restrict__ Y, const float* __restrict
     if ((threadIdx.x & 63) == 0 && e2 != 0.0f) atomicAdd(loss, e2 / (3.0f * (float)M));
 }
 
-// dZ = dA * silu'(Z), as BF16 (and FP32 when dZf is given)
-__global__ void silu_bwd_kernel(const float* __restrict__ dA, const float* __restrict__ Z,
-                                bf16* __restrict__ dZb, float* __restrict__ dZf, long long n)
+// dZ = dA * act'(Z), as BF16 (and FP32 when dZf is given)
+__global__ void act_bwd_kernel(const float* __restrict__ dA, const float* __restrict__ Z,
+                               bf16* __restrict__ dZb, float* __restrict__ dZf, long long n)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const float z = Z[t], s = sigmoidf_(z);
-    const float g = dA[t] * s * (1.0f + z * (1.0f - s));
+    const float g = dA[t] * act_grad(Z[t]);
     if (dZb) dZb[t] = f2bf(g);
     if (dZf) dZf[t] = g;
 }
@@ -444,8 +451,8 @@ __device__ __forceinline__ void hidden_layer(const bf16* __restrict__ W,
         for (int t = 0; t < 16; t++) {
             const int n0 = 32 * np + 8 * (t >> 2) + 4 * h + (t & 3);
             const int g = 2 * np + (t >> 3), e = 4 * ((t >> 2) & 1) + (t & 3);
-            hn[g][e] = (short)f2bf_d(siluf_(acc0[t] + bias[n0]));
-            hn[g + 2][e] = (short)f2bf_d(siluf_(acc1[t] + bias[n0 + 32]));
+            hn[g][e] = (short)f2bf_d(act_(acc0[t] + bias[n0]));
+            hn[g + 2][e] = (short)f2bf_d(act_(acc1[t] + bias[n0 + 32]));
         }
     }
 #pragma unroll
@@ -464,7 +471,7 @@ __device__ __forceinline__ void hidden_layer(const bf16* __restrict__ W,
 // Occupancy: the weights take 81.5 KB of LDS, so two workgroups fit a CU; with
 // 8 waves per workgroup and at most 128 registers per lane (4 waves per SIMD,
 // the launch bound) that is 16 waves per CU, twice the 4-wave workgroup's
-// count -- the MFMA chains of one wave and the SiLUs of another overlap.
+// count -- the MFMA chains of one wave and the activations of another overlap.
 constexpr int kApplyWaves = 8;                  // waves per workgroup, 32 atoms each
 constexpr int kApplyThreads = 64 * kApplyWaves;
 constexpr int kApplyAtoms = 32 * kApplyWaves;
@@ -522,7 +529,7 @@ __global__ __launch_bounds__(kApplyThreads, 4) void apply_fused_kernel(
                 for (int i = 0; i < 4; i++) {
                     const float4 w = sW1[n + i];
                     hb[s][4 * half + i] =
-                        (short)f2bf_d(siluf_(layer1_offsets(cv[i], w.x, w.y, w.z, d0, d1, d2)));
+                        (short)f2bf_d(act_(layer1_offsets(cv[i], w.x, w.y, w.z, d0, d1, d2)));
                 }
             }
         }
@@ -675,19 +682,19 @@ void backward(disco_surrogate* s, int M, hipStream_t st)
     colsum(s, st, s->dY, nullptr, s->G + oB4, M, kOut);
     gemm<0>(st, s->dY, kOut, 1, W4, kH, 1, M, kH, kOut, nullptr, s->dA, kH, nullptr, 0);
     // layer 3
-    hipLaunchKernelGGL(silu_bwd_kernel, dim3(blocks((long long)M * kH, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(act_bwd_kernel, dim3(blocks((long long)M * kH, 256)), dim3(256), 0, st,
                        s->dA, s->Z3, s->dZ, (float*)nullptr, (long long)M * kH);
     gemm_wgrad(s, st, s->dZ, 1, kH, s->A2, kH, 1, kH, kH, M, s->G + oW3);
     colsum(s, st, s->dZ, nullptr, s->G + oB3, M, kH);
     gemm<0>(st, s->dZ, kH, 1, W3, kH, 1, M, kH, kH, nullptr, s->dA, kH, nullptr, 0);
     // layer 2
-    hipLaunchKernelGGL(silu_bwd_kernel, dim3(blocks((long long)M * kH, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(act_bwd_kernel, dim3(blocks((long long)M * kH, 256)), dim3(256), 0, st,
                        s->dA, s->Z2, s->dZ, (float*)nullptr, (long long)M * kH);
     gemm_wgrad(s, st, s->dZ, 1, kH, s->A1, kH, 1, kH, kH, M, s->G + oW2);
     colsum(s, st, s->dZ, nullptr, s->G + oB2, M, kH);
     gemm<0>(st, s->dZ, kH, 1, W2, kH, 1, M, kH, kH, nullptr, s->dA, kH, nullptr, 0);
     // layer 1 (FP32)
-    hipLaunchKernelGGL(silu_bwd_kernel, dim3(blocks((long long)M * kH, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(act_bwd_kernel, dim3(blocks((long long)M * kH, 256)), dim3(256), 0, st,
                        s->dA, s->Z1, (bf16*)nullptr, s->dZf, (long long)M * kH);
     {
         const int S = (M + kSlice - 1) / kSlice;

@@ -14,7 +14,7 @@ is the far shell.  Its surrogate is
   moments -- exact through degree 5 for cuts whose moments are isotropic up
   to rank 5 (circles, regular hexagons), so its far-shell error is
   O((R/r)^6);
-* ``MLP`` -- 14 -> 128 -> 128 -> 128 -> 3 (SiLU), trained by the library's own
+* ``MLP`` -- 14 -> 128 -> 128 -> 128 -> 3 (hard swish), trained by the library's own
   GPU kernels (``disco_hip_surrogate_*``) on exact far shells generated by the
   line kernels, with ``scale = |b| (R / L)^4``.
 

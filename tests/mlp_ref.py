@@ -33,8 +33,14 @@ def pack(d):
     return np.concatenate([d[name].reshape(-1) for name, _ in SIZES])
 
 
-def _sig(z):
-    return 1.0 / (1.0 + np.exp(-z))
+def _act(z):
+    """Hard swish: z clip(z/6 + 1/2, 0, 1)."""
+    return z * np.clip(z / 6.0 + 0.5, 0.0, 1.0)
+
+
+def _act_grad(z):
+    g = z / 6.0 + 0.5
+    return np.where(g <= 0.0, 0.0, np.where(g >= 1.0, 1.0, z / 3.0 + 0.5))
 
 
 def forward(p, X14, rnd=True):
@@ -44,11 +50,11 @@ def forward(p, X14, rnd=True):
     r = bf16 if rnd else (lambda a: np.asarray(a, np.float64))
     c = {"X": X}
     c["Z1"] = X @ P["W1"].T + P["b1"]
-    c["A1"] = r(c["Z1"] * _sig(c["Z1"]))
+    c["A1"] = r(_act(c["Z1"]))
     c["Z2"] = c["A1"] @ r(P["W2"]).T + P["b2"]
-    c["A2"] = r(c["Z2"] * _sig(c["Z2"]))
+    c["A2"] = r(_act(c["Z2"]))
     c["Z3"] = c["A2"] @ r(P["W3"]).T + P["b3"]
-    c["A3"] = r(c["Z3"] * _sig(c["Z3"]))
+    c["A3"] = r(_act(c["Z3"]))
     c["Y"] = c["A3"] @ r(P["W4"]).T + P["b4"]
     return c["Y"][:, :3], c
 
@@ -67,18 +73,15 @@ def loss_grad(p, X14, T, rnd=True):
     g["b4"] = dY.sum(0)
     dA = dY @ r(P["W4"])
 
-    def silu_grad(Z):
-        s = _sig(Z)
-        return s * (1.0 + Z * (1.0 - s))
-    dZ = r(dA * silu_grad(c["Z3"]))
+    dZ = r(dA * _act_grad(c["Z3"]))
     g["W3"] = dZ.T @ c["A2"]
     g["b3"] = dZ.sum(0)
     dA = dZ @ r(P["W3"])
-    dZ = r(dA * silu_grad(c["Z2"]))
+    dZ = r(dA * _act_grad(c["Z2"]))
     g["W2"] = dZ.T @ c["A1"]
     g["b2"] = dZ.sum(0)
     dA = dZ @ r(P["W2"])
-    dZ1 = dA * silu_grad(c["Z1"])
+    dZ1 = dA * _act_grad(c["Z1"])
     g["W1"] = dZ1.T @ c["X"]
     g["b1"] = dZ1.sum(0)
     g["W4"][3:] = 0.0
