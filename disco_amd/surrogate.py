@@ -387,9 +387,9 @@ def base_far_shell_torch(ctx, x, y, z, loops, box, img, kind="rule7", u=None,
     origin, dims, nodes = grid_nodes(x, y, z, grid_h)
     # a grid holds few nodes (tens of thousands) for the GPU's 256 CUs: the loops
     # are split into contiguous groups, each summed by its own workgroups of one
-    # launch, then added in group order (about 2,048 workgroups in all)
+    # launch, then added in group order (about 8,192 workgroups in all: 35k waves)
     tiles = (nodes[0].numel() + 255) // 256
-    want = grid_groups if grid_groups else -(-2048 // tiles)
+    want = grid_groups if grid_groups else -(-8192 // tiles)
     groups = max(1, min(want, len(base), 256))
     plan = B.Plan(ctx, base, box, img, prec, kind="loopon")
     set_inner(plan, 1)
