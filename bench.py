@@ -37,21 +37,61 @@ FP32_UNPACKED_PEAK_TFLOPS = 78.6  # the same without packed FP32 (one FMA per la
 # (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, each scaled by a plain copy of
 # known size, `--workload copy`): 193.9e6 B per dispatch over 4,000,000 atoms.
 FAR_PASS_HBM_BYTES_PER_ATOM = 48.49
+TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on this workload: "
+                  "profiles/line_kernel_far__fcc4m_50loops_img2__pmc_traffic.md")
 
 
-def cpu_baseline(sample, stride):
-    """FP64 oracle (the reference's loop structure, OpenMP) on every `stride`-th atom."""
+def host_cpu():
+    """CPU model, physical cores and the CPUs this process may run on (the
+    host's /proc/cpuinfo and the scheduler affinity)."""
+    model, cores = None, set()
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"cpu_model": model, "host_physical_cores": len(cores) or None,
+            "host_logical_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
+def cpu_baseline(sample, stride, runs=3):
+    """FP64 oracle (the reference's loop structure, OpenMP) on every `stride`-th
+    atom: the median of `runs` timed runs."""
     import oracle
     idx = np.arange(0, sample.n, stride)
     x, y, z = sample.x[idx], sample.y[idx], sample.z[idx]
-    t0 = time.perf_counter()
-    oracle.line_sum(x, y, z, sample.loops, sample.box, sample.img)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        oracle.line_sum(x, y, z, sample.loops, sample.box, sample.img)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     evals = idx.size * sample.segments() * (2 * sample.img + 1) ** 3
-    return {"value": evals / dt, "unit": "evals/s", "cores": oracle.threads(), "kind": "port",
-            "sample": f"every {stride}th atom ({idx.size} atoms) of the same workload, all "
-                      f"{len(sample.loops)} loops, images +-{sample.img}; {dt:.1f} s",
-            "seconds": dt}
+    out = {"value": evals / dt, "unit": "evals/s", "cores": oracle.threads(), "kind": "port",
+           "sample": f"every {stride}th atom ({idx.size} atoms) of the same workload, all "
+                     f"{len(sample.loops)} loops, images +-{sample.img}; median of {runs} "
+                     f"runs, {dt:.1f} s each",
+           "seconds": dt, "runs": times, "threads": oracle.threads()}
+    out.update(host_cpu())
+    return out
 
 
 def copy_reference(args):
@@ -82,9 +122,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-stride", type=int, default=128)
+    ap.add_argument("--cpu-stride", type=int, default=256)
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--workload", default="c2",
                     choices=["c2", "c3", "c4", "c5", "copy", "lattice", "writers", "prismatic",
                              "apply"],
@@ -106,7 +148,7 @@ def main():
         copy_reference(args)
         return
     if args.workload != "c2":
-        from disco_amd import bench_workloads
+        import bench_workloads
         bench_workloads.run(args)
         return
 
@@ -151,31 +193,51 @@ def main():
     ms_step = total_ms / args.steps
     value = evals / (ms_step * 1e-3)
 
-    # accuracy of the timed output against the GPU FP64 sum (every 8th atom)
+    # accuracy of the timed output against the GPU FP64 sum, over every atom
     err = None
     fp64 = None
     if not args.no_check:
-        idx = torch.arange(0, s.n, 8, device=dev)
-        xs, ys, zs = x[idx].contiguous(), y[idx].contiguous(), z[idx].contiguous()
-        vx, vy, vz = (torch.zeros_like(xs) for _ in range(3))
+        v = [torch.zeros_like(x) for _ in range(3)]
         p64 = B.Plan(ctx, s.loops, s.box, s.img, B.PREC_FP64)
         p64.profile(True)
-        p64.run_torch(xs, ys, zs, vx, vy, vz)
+        p64.run_torch(x, y, z, *v)
         ms64 = p64.kernel_ms()[0]
-        d = torch.stack([ux[idx] - vx, uy[idx] - vy, uz[idx] - vz]).abs().max().item()
-        err = d
-        fp64 = {"atoms": int(idx.numel()), "ms": ms64,
-                "evals_per_s": idx.numel() * plan.evals_per_atom() / (ms64 * 1e-3)}
+        err = max(float((a - b).abs().max()) for a, b in zip((ux, uy, uz), v))
+        fp64 = {"atoms": s.n, "ms": ms64, "evals_per_s": evals / (ms64 * 1e-3)}
         p64.free()
-        print(f"[bench] max|du| vs FP64 (every 8th atom): {err:.3e} A; FP64 mode on those "
-              f"{idx.numel()} atoms: {ms64:.1f} ms ({fp64['evals_per_s']:.3e} evals/s)",
-              flush=True)
+        del v
+        print(f"[bench] max|du| vs FP64 over all {s.n} atoms: {err:.3e} A; FP64 mode: "
+              f"{ms64:.1f} ms ({fp64['evals_per_s']:.3e} evals/s)", flush=True)
+
+    # kroupaGPU end to end: host arrays in, plan + upload + kernels + download +
+    # accumulation into the caller's u out (Code/backend_cuda.cu:568-722 times the
+    # same phases)
+    e2e = None
+    if not args.no_e2e:
+        runs = []
+        hu = [np.zeros(s.n) for _ in range(3)]
+        for k in range(3):
+            for a in hu:
+                a[:] = 0.0
+            t = B.BackendTiming()
+            t0 = time.perf_counter()
+            B.line_sum(ctx, s.x, s.y, s.z, s.loops, s.box, s.img, B.PREC_FP32C, u=hu, timing=t)
+            runs.append({"wall_ms": (time.perf_counter() - t0) * 1e3, "upload_ms": t.upload * 1e3,
+                         "kernels_ms": t.kernels * 1e3, "download_ms": t.download * 1e3})
+            print(f"[bench] end to end {k + 1}/3: {runs[-1]['wall_ms']:.1f} ms (upload "
+                  f"{runs[-1]['upload_ms']:.1f}, kernels {runs[-1]['kernels_ms']:.1f}, download "
+                  f"{runs[-1]['download_ms']:.1f})", flush=True)
+        med = sorted(runs, key=lambda r: r["wall_ms"])[1]
+        e2e = dict(med, runs=runs,
+                   max_abs_diff_vs_device_run_A=max(float(np.abs(a - b.cpu().numpy()).max())
+                                                    for a, b in zip(hu, (ux, uy, uz))))
 
     far = float(np.mean(far_ms))
     achieved = evals * FLOP_PER_EVAL / (far * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
                 "traffic": FAR_PASS_HBM_BYTES_PER_ATOM * s.n, "traffic_unit": "bytes",
+                "traffic_source": TRAFFIC_SOURCE,
                 "peak_unpacked": FP32_UNPACKED_PEAK_TFLOPS,
                 "frac_unpacked": achieved / FP32_UNPACKED_PEAK_TFLOPS,
                 "kernel": "line_kernel<PASS_FAR, LOOP_GLIDE>", "kernel_ms": far,
@@ -184,7 +246,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(s, args.cpu_stride)
+        cpu = cpu_baseline(s, args.cpu_stride, args.cpu_runs)
         print(f"[bench] cpu baseline: {cpu['value']:.3e} evals/s on {cpu['cores']} threads",
               flush=True)
     out = {"metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1,
@@ -197,6 +259,7 @@ def main():
                       "atoms": s.n, "loops": len(s.loops), "segments": s.segments(),
                       "images": (2 * s.img + 1) ** 3, "precision": "FP32C", "seed": 20241,
                       "evals_per_step": evals},
+           "e2e_ms": e2e["wall_ms"] if e2e else None, "e2e": e2e, "max_abs_err_atoms": s.n,
            "roofline": roofline, "cpu_baseline": cpu, "fp64_mode": fp64, "wall_s": wall}
     print(json.dumps(out), flush=True)
     plan.free()

@@ -44,8 +44,8 @@ def _cpu_line(sample, stride, img=None):
 def run_c3(args):
     import torch
     import oracle
-    from . import backend as B
-    from . import geometry as G
+    from disco_amd import backend as B
+    from disco_amd import geometry as G
     s = G.config_c3()
     dev = torch.device("cuda:0")
     x, y, z = (torch.from_numpy(a).to(dev) for a in (s.x, s.y, s.z))
@@ -99,8 +99,8 @@ def run_c3(args):
 
 def run_c4(args):
     import torch
-    from . import backend as B
-    from . import geometry as G
+    from disco_amd import backend as B
+    from disco_amd import geometry as G
     ctx = B.Context(0)
     G._nlib()
     # sample generation: loop contours by the C++ producer, lattices on the device
@@ -186,9 +186,9 @@ def run_c4(args):
 
 def run_c5(args):
     import torch
-    from . import backend as B
-    from . import geometry as G
-    from . import surrogate as S
+    from disco_amd import backend as B
+    from disco_amd import geometry as G
+    from disco_amd import surrogate as S
     img = 4
     t0 = time.time()
     c = G.config_c2(seed=20245, nloops=200)
@@ -296,8 +296,8 @@ def run_lattice(args):
     of its positions to the device."""
     import torch
     import oracle
-    from . import backend as B
-    from . import geometry as G
+    from disco_amd import backend as B
+    from disco_amd import geometry as G
     kind, nc, a0 = "FCC", 100, 3.615
     ctx = B.Context(0)
     n = B.lattice_atoms(kind, nc, nc, nc)
@@ -362,9 +362,9 @@ def run_writers(args):
     import tempfile
     import torch
     import oracle
-    from . import backend as B
-    from . import geometry as G
-    from . import writers as W
+    from disco_amd import backend as B
+    from disco_amd import geometry as G
+    from disco_amd import writers as W
     s = G.config_c2()
     ctx = B.Context(0)
     dev = torch.device("cuda:0")
@@ -478,7 +478,7 @@ def prismatic_loops(a0, cells, count=20, seed=7):
     """The prismatic-slab workload's loops: FCC {110} interstitial / vacancy loops
     (b = a0/2<110> along n: 2 planes), circles, rounded heptagons and rounded
     rectangles of 20-60 A at random snapped centres."""
-    from . import geometry as G
+    from disco_amd import geometry as G
     rng = np.random.default_rng(seed)
     normals = [[1, 1, 0], [1, -1, 0], [0, 1, 1], [0, 1, -1], [1, 0, 1], [1, 0, -1]]
     out = []
@@ -506,8 +506,8 @@ def run_prismatic(args):
     restatement of the reference's host scan (one thread, as the reference)."""
     import torch
     import oracle
-    from . import backend as B
-    from . import prismatic as P
+    from disco_amd import backend as B
+    from disco_amd import prismatic as P
     a0, cells = 3.615, 100
     ctx = B.Context(0)
     x, y, z = B.lattice_torch(ctx, "FCC", cells, cells, cells, a0)
@@ -617,8 +617,8 @@ def run_apply(args):
     GEMMs and the FP64 restatement is tested; no host baseline is timed here (the
     workload that uses it, c5, times the plain sum beside it)."""
     import torch
-    from . import backend as B
-    from . import surrogate as S
+    from disco_amd import backend as B
+    from disco_amd import surrogate as S
     ctx = B.Context(0)
     x, y, z = B.lattice_torch(ctx, "FCC", 100, 100, 100, 3.615)
     box = [361.5] * 3

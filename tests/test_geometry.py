@@ -3,6 +3,7 @@ geometry (golden outputs of Code/loop.cpp and Code/lattice.hpp)."""
 import numpy as np
 import pytest
 
+import oracle
 from conftest import golden
 from disco_amd import geometry as G
 
@@ -69,6 +70,18 @@ def test_lattice_is_bit_identical_to_reference():
     for lat, a0 in (("BCC", 3.0), ("FCC", 3.615)):
         x, y, z = G.lattice(lat, 3, 4, 5, a0)
         assert np.array_equal(np.stack([x, y, z]), g[f"lattice_{lat}"])
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference build not present")
+@pytest.mark.parametrize("lat,a0,cells", [("BCC", 2.8553, 40), ("BCC", 2.8553, 60),
+                                          ("FCC", 3.615, 100)])
+def test_lattice_bit_identical_to_compiled_reference_at_workload_sizes(lat, a0, cells):
+    """The lattices of the 128,000-, 432,000- and 4,000,000-atom workloads against the
+    reference's generateLattice (Code/lattice.hpp:35-51) compiled from its sources:
+    the device lattice kernel is tested against G.lattice at these sizes."""
+    x, y, z = G.lattice(lat, cells, cells, cells, a0)
+    rx, ry, rz = oracle.ref_lattice(lat, cells, cells, cells, a0)
+    assert np.array_equal(x, rx) and np.array_equal(y, ry) and np.array_equal(z, rz)
 
 
 def test_c1_loop_centres_match_the_measured_snap():

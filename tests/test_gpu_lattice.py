@@ -12,6 +12,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("kind,n,a0", [("BCC", (3, 4, 5), 2.8553), ("FCC", (5, 3, 4), 3.615),
                                        ("BCC", (40, 40, 40), 2.8553),
+                                       ("BCC", (60, 60, 60), 2.8553),
                                        ("FCC", (100, 100, 100), 3.615),
                                        ("BCC", (60, 1, 37), 2.8665), ("FCC", (1, 1, 1), 4.05)])
 def test_lattice_bit_identical(hip_ctx, kind, n, a0):
