@@ -155,6 +155,7 @@ def lib():
         L.disco_hip_plan_loopon.argtypes = [_vp, ctypes.POINTER(_LooponLoop), _i, _d, _d, _d, _i,
                                             _i, ctypes.POINTER(_vp)]
         L.disco_hip_plan_run.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]
+        L.disco_hip_plan_run_noalloc.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]
         L.disco_hip_plan_profile.argtypes = [_vp, _i]
         L.disco_hip_plan_kernel_ms.argtypes = [_vp, _dp, _i]
         L.disco_hip_plan_evals_per_atom.argtypes = [_vp]

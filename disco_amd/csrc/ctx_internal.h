@@ -31,6 +31,9 @@ struct disco_ctx {
     int text_profile = 0;
     hipEvent_t text_ev[4] = {nullptr, nullptr, nullptr, nullptr};
     float text_ms[3] = {0.f, 0.f, 0.f};
+    // this device's copy of the library's static scratch array (the address
+    // hipGetSymbolAddress gives for it), used by disco_hip_plan_run_noalloc
+    void* noalloc_scratch = nullptr;
 };
 
 namespace disco {

@@ -621,6 +621,35 @@ __global__ __launch_bounds__(BLOCK) void batch_line_kernel(
 
 }  // namespace
 
+// Scratch of disco_hip_plan_run_noalloc: a static device array (each device
+// holds its own copy for the life of the process).  Host code never passes
+// its name to a kernel: the name is a host-side stand-in, and the device
+// address of this device's copy comes from hipGetSymbolAddress, taken once per
+// device by disco_hip_open and kept in noalloc_scratch_addr.
+static __device__ __align__(256) double scratch[DISCO_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// The device address of `device`'s copy of scratch; `device` must be current.
+hipError_t noalloc_scratch_addr(int device, void** out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[device] = p;
+    }
+    *out = g_scratch_addr[device];
+    return hipSuccess;
+}
+}  // namespace
+
 struct disco_plan {
     disco_ctx* ctx = nullptr;
     int kind = 0;          // 0 line, 1 loopon
@@ -716,11 +745,22 @@ void free_plan(disco_plan* p)
     delete p;
 }
 
+// scratch = nullptr: a split loopon plan keeps its partial sums in its own
+// buffer (allocated, grow-only); otherwise they go to `scratch`, which holds
+// scratch_bytes (a larger need fails before anything is launched).
 int launch(disco_plan* p, int n, const double* x, const double* y, const double* z, double* ux,
-           double* uy, double* uz, int accumulate, hipStream_t s)
+           double* uy, double* uz, int accumulate, hipStream_t s, double* scratch = nullptr,
+           size_t scratch_bytes = 0)
 {
     disco_ctx* ctx = p->ctx;
     if (n == 0) return DISCO_OK;
+    const int split = p->split < p->nloops ? p->split : p->nloops;
+    const size_t split_need = (size_t)3 * (size_t)n * (size_t)split * sizeof(double);
+    if (p->kind == 1 && split > 1 && scratch && split_need > scratch_bytes)
+        return fail(ctx, DISCO_ERR_TOO_LARGE,
+                    "partial sums of " + std::to_string(split_need) + " bytes (24 B x atoms x " +
+                        "loop groups) exceed the " + std::to_string(scratch_bytes) +
+                        "-byte static scratch");
     const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
     const bool prof = p->profile && p->nloops > 0;
     p->profiled_run = false;
@@ -767,27 +807,29 @@ int launch(disco_plan* p, int n, const double* x, const double* y, const double*
         const LooponLoop* lp = (const LooponLoop*)p->d_loops;
         const float4* pf = (const float4*)p->d_f;
         const double4* pd = (const double4*)p->d_d;
-        const int split = p->split < p->nloops ? p->split : p->nloops;
         if (split > 1) {
-            const size_t need = (size_t)3 * (size_t)n * (size_t)split * sizeof(double);
-            if (p->cap_split < need) {
-                if (p->d_split) DISCO_HIP_TRY(ctx, hipFree(p->d_split));
-                p->d_split = nullptr;
-                p->cap_split = 0;
-                DISCO_HIP_TRY(ctx, hipMalloc(&p->d_split, need));
-                p->cap_split = need;
+            double* partial = scratch;
+            if (!partial) {
+                if (p->cap_split < split_need) {
+                    if (p->d_split) DISCO_HIP_TRY(ctx, hipFree(p->d_split));
+                    p->d_split = nullptr;
+                    p->cap_split = 0;
+                    DISCO_HIP_TRY(ctx, hipMalloc(&p->d_split, split_need));
+                    p->cap_split = split_need;
+                }
+                partial = p->d_split;
             }
             const dim3 g2(grid.x, (unsigned)split);
             if (p->prec == DISCO_PREC_FP64)
                 hipLaunchKernelGGL(loopon_split_kernel<DISCO_PREC_FP64>, g2, block, 0, s, x, y, z,
                                    n, lp, p->nloops, pf, pd, p->L[0], p->L[1], p->L[2], p->img,
-                                   p->inner, split, p->d_split);
+                                   p->inner, split, partial);
             else
                 hipLaunchKernelGGL(loopon_split_kernel<DISCO_PREC_FP32C>, g2, block, 0, s, x, y,
                                    z, n, lp, p->nloops, pf, pd, p->L[0], p->L[1], p->L[2], p->img,
-                                   p->inner, split, p->d_split);
+                                   p->inner, split, partial);
             DISCO_HIP_TRY(ctx, hipGetLastError());
-            hipLaunchKernelGGL(loopon_split_sum_kernel, grid, block, 0, s, p->d_split, n, split,
+            hipLaunchKernelGGL(loopon_split_sum_kernel, grid, block, 0, s, partial, n, split,
                                ux, uy, uz, accumulate);
         } else if (p->prec == DISCO_PREC_FP64)
             hipLaunchKernelGGL(loopon_kernel<DISCO_PREC_FP64>, grid, block, 0, s, x, y, z, ux, uy,
@@ -1174,7 +1216,9 @@ int disco_hip_open(int device, disco_ctx** out)
     if (!ctx) return DISCO_ERR_OOM;
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
-        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        noalloc_scratch_addr(device, &ctx->noalloc_scratch) != hipSuccess ||
+        !ctx->noalloc_scratch) {
         disco_hip_close(ctx);
         return DISCO_ERR_HIP;
     }
@@ -1269,6 +1313,22 @@ int disco_hip_plan_run(disco_plan* plan, int n, const double* d_x, const double*
     DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
     return launch(plan, n, d_x, d_y, d_z, d_ux, d_uy, d_uz, accumulate ? 1 : 0,
                   (hipStream_t)stream);
+}
+
+int disco_hip_plan_run_noalloc(disco_plan* plan, int n, const double* d_x, const double* d_y,
+                               const double* d_z, double* d_ux, double* d_uy, double* d_uz,
+                               int accumulate, void* stream)
+{
+    if (!plan) return DISCO_ERR_ARG;
+    disco_ctx* ctx = plan->ctx;
+    if (n < 0) return fail(ctx, DISCO_ERR_ARG, "negative atom count");
+    if (n > 0 && (!d_x || !d_y || !d_z || !d_ux || !d_uy || !d_uz))
+        return fail(ctx, DISCO_ERR_ARG, "null device array");
+    if (!ctx->noalloc_scratch) return fail(ctx, DISCO_ERR_STATE, "no static scratch address");
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    return launch(plan, n, d_x, d_y, d_z, d_ux, d_uy, d_uz, accumulate ? 1 : 0,
+                  (hipStream_t)stream, (double*)ctx->noalloc_scratch,
+                  (size_t)DISCO_NOALLOC_SCRATCH_BYTES);
 }
 
 int disco_hip_plan_profile(disco_plan* plan, int on)

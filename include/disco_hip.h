@@ -139,6 +139,23 @@ int disco_hip_plan_loopon(disco_ctx *ctx, const disco_loopon_loop *loops, int nl
 int disco_hip_plan_run(disco_plan *plan, int n, const double *d_x, const double *d_y,
                        const double *d_z, double *d_ux, double *d_uy, double *d_uz,
                        int accumulate, void *stream);
+/* (No counterpart in the reference.)  disco_hip_plan_run for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc: the same kernels and results, and no
+ * allocation in the call.  The partial sums of a split loopon plan
+ * (disco_hip_plan_set_split: 24 B x n x groups), which disco_hip_plan_run keeps
+ * in a buffer the plan allocates, go to a static device array of
+ * DISCO_NOALLOC_SCRATCH_BYTES that the library holds on each device until the
+ * process ends; a larger need fails with DISCO_ERR_TOO_LARGE before anything is
+ * launched.  Plans without scratch (line plans, unsplit loopon plans) run as
+ * in disco_hip_plan_run.  Every call on one device shares that device's array
+ * and the kernels write every partial sum they read, so nothing carries over
+ * between calls; calls that overlap on one device (other streams or threads)
+ * race and must be serialised by the caller. */
+#define DISCO_NOALLOC_SCRATCH_BYTES (64u << 20)
+int disco_hip_plan_run_noalloc(disco_plan *plan, int n, const double *d_x, const double *d_y,
+                               const double *d_z, double *d_ux, double *d_uy, double *d_uz,
+                               int accumulate, void *stream);
 /* Per-kernel HIP-event timing of later runs (on = 1; off by default).  A
  * profiled run records events on its stream around each of its kernels. */
 int disco_hip_plan_profile(disco_plan *plan, int on);
