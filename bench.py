@@ -37,6 +37,11 @@ FP32_UNPACKED_PEAK_TFLOPS = 78.6  # the same without packed FP32 (one FMA per la
 # (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, each scaled by a plain copy of
 # known size, `--workload copy`): 193.9e6 B per dispatch over 4,000,000 atoms.
 FAR_PASS_HBM_BYTES_PER_ATOM = 48.49
+# cubature form of the sample (reported beside the plain sum): exact within 5 cut
+# radii of a loop image, the 9-point cubature beyond; +-1 images far from every
+# atom on an L/64 grid (interpolation error bound 1e-7 A each), the far shell on
+# an L/24 grid
+K_EXACT, NEAR_TOL = 5.0, 1e-7
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on this workload: "
                   "profiles/line_kernel_far__fcc4m_50loops_img2__pmc_traffic.md")
 
@@ -127,6 +132,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-e2e", action="store_true")
+    ap.add_argument("--no-cubature", action="store_true")
+    ap.add_argument("--c5-all", action="store_true",
+                    help="c5: check the error on every atom (about 50 s of FP64) instead of "
+                         "every 8th")
     ap.add_argument("--workload", default="c2",
                     choices=["c2", "c3", "c4", "c5", "copy", "lattice", "writers", "prismatic",
                              "apply"],
@@ -196,6 +205,7 @@ def main():
     # accuracy of the timed output against the GPU FP64 sum, over every atom
     err = None
     fp64 = None
+    v = None
     if not args.no_check:
         v = [torch.zeros_like(x) for _ in range(3)]
         p64 = B.Plan(ctx, s.loops, s.box, s.img, B.PREC_FP64)
@@ -205,9 +215,38 @@ def main():
         err = max(float((a - b).abs().max()) for a, b in zip((ux, uy, uz), v))
         fp64 = {"atoms": s.n, "ms": ms64, "evals_per_s": evals / (ms64 * 1e-3)}
         p64.free()
-        del v
         print(f"[bench] max|du| vs FP64 over all {s.n} atoms: {err:.3e} A; FP64 mode: "
               f"{ms64:.1f} ms ({fp64['evals_per_s']:.3e} evals/s)", flush=True)
+
+    # the same sample in cubature form (exact within K_EXACT loop radii of a loop
+    # image, the 7-point cubature beyond, far images on grids): ms per sample and
+    # its error against the FP64 plain sum over every atom
+    cub = None
+    if not args.no_cubature:
+        from disco_amd import cubature as C
+        cs = C.CubatureSum(ctx, x, y, z, s.loops, s.box, s.img, K_EXACT, near_tol=NEAR_TOL)
+        w = [torch.zeros_like(x) for _ in range(3)]
+        cs.run(x, y, z, *w)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            cs.run(x, y, z, *w)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        cms = e0.elapsed_time(e1) / args.steps
+        cerr = max(float((a - b).abs().max()) for a, b in zip(w, v)) if v is not None else None
+        cub = {"ms_per_sample": cms, "max_abs_err_A": cerr, "err_atoms": s.n if v is not None else None,
+               "speedup_vs_plain": ms_step / cms, "k_exact": K_EXACT, "near_tol_A": NEAR_TOL,
+               "rule_loops": cs.eligible, "images_on_grid": cs.skipped,
+               "form": "exact line integral within k_exact R of each loop image (FP32, FP64 "
+                       "near slab), 9-point cubature of the cut beyond; +-1 images far from "
+                       "every atom on an L/64 grid, the far shell on an L/24 grid"}
+        cs.free()
+        del w
+        print(f"[bench] cubature form: {cms:.1f} ms per sample ({ms_step / cms:.1f}x the plain "
+              f"sum), max|du| vs FP64 over all atoms {cerr:.3e} A", flush=True)
+    del v
 
     # kroupaGPU end to end: host arrays in, plan + upload + kernels + download +
     # accumulation into the caller's u out (Code/backend_cuda.cu:568-722 times the
@@ -260,6 +299,7 @@ def main():
                       "images": (2 * s.img + 1) ** 3, "precision": "FP32C", "seed": 20241,
                       "evals_per_step": evals},
            "e2e_ms": e2e["wall_ms"] if e2e else None, "e2e": e2e, "max_abs_err_atoms": s.n,
+           "cubature": cub,
            "roofline": roofline, "cpu_baseline": cpu, "fp64_mode": fp64, "wall_s": wall}
     print(json.dumps(out), flush=True)
     plan.free()

@@ -8,9 +8,9 @@ Each prints one JSON line in bench.py's format.  The default bench line (the
 * c4 -- batched datagen: 256 BCC samples (20-60 cells, 1-16 loops of 16-128
         segments: circles, rectangles, parallelograms), images +-1, one
         persistent launch per pass replayed as a HIP graph, autotuned shape.
-* c5 -- learned far shell: FCC 100^3 (4M atoms), 200 loops as the default
-        sample, images +-4: exact line integral for +-1, surrogate (dipole far
-        shell + MLP) for the rest, validated against the exact far shell.
+* c5 -- images +-4: FCC 100^3 (4M atoms), 200 loops as the default sample,
+        in cubature form (exact near each loop image, a 7-point cubature of the
+        cut far from it, far images on grids), against the FP64 plain sum.
 """
 from __future__ import annotations
 
@@ -185,107 +185,110 @@ def run_c4(args):
 
 
 def run_c5(args):
+    """The 200-loop sample at images +-4 (729 per loop) in cubature form: exact
+    line integral within 5 cut radii of each loop image (FP32, FP64 near slab),
+    the 9-point cubature of the cut beyond, the +-1 images far from every atom
+    on an L/64 grid and the far shell (images 2..4) on an L/24 grid.
+    Error against the FP64 plain sum on every 8th atom (every atom with
+    --c5-all); the plain GPU sum timed on a contiguous eighth of the atoms and
+    scaled; the oracle timed on a few atoms."""
     import torch
     from disco_amd import backend as B
+    from disco_amd import cubature as C
     from disco_amd import geometry as G
-    from disco_amd import surrogate as S
-    img = 4
+    img, k_exact, near_tol = 4, 5.0, 1e-7
     t0 = time.time()
     c = G.config_c2(seed=20245, nloops=200)
-    infos = [S.shape_info(lp, g[1], "circle" if g[0] == 0 else "polygon")
-             for lp, g in zip(c.loops, c.meta["geo"])]
     print(f"[c5] workload {c.n} atoms, {len(c.loops)} loops, {c.segments()} segments "
           f"({time.time() - t0:.1f} s)", flush=True)
     dev = torch.device("cuda:0")
     x, y, z = (torch.from_numpy(a).to(dev) for a in (c.x, c.y, c.z))
     ctx = B.Context(0)
-    # ---- training (one-time), for both baselines
-    train = {}
-    surs = {}
-    for base in ("dipole", "rule7"):
-        t0 = time.perf_counter()
-        F, T, Sc, ex, bs = S.make_training_set(ctx, nloops=args.train_loops, atoms_per_loop=500,
-                                               img=img, seed=11, baseline=base)
-        t_data = time.perf_counter() - t0
-        n = F.shape[0]
-        ntr = int(0.9 * n)
-        sur = S.Surrogate(ctx, seed=1)
-        t0 = time.perf_counter()
-        loss = sur.train(F[:ntr], T[:ntr], epochs=args.epochs, batch=4096, lr=2e-3)
-        t_train = time.perf_counter() - t0
-        pred = sur.forward(F[ntr:])
-        err_net = float(np.abs((pred - T[ntr:]) * Sc[ntr:, None]).max())
-        err_base = float(np.abs(T[ntr:] * Sc[ntr:, None]).max())
-        rms_net = float(np.sqrt((((pred - T[ntr:]) * Sc[ntr:, None]) ** 2).mean()))
-        rms_base = float(np.sqrt(((T[ntr:] * Sc[ntr:, None]) ** 2).mean()))
-        train[base] = {"pairs": n, "data_s": t_data, "train_s": t_train, "epochs": args.epochs,
-                       "final_loss": loss, "held_out_max_err_A": err_net,
-                       "held_out_base_only_max_A": err_base, "held_out_rms_A": rms_net,
-                       "held_out_base_only_rms_A": rms_base}
-        surs[base] = sur
-        print(f"[c5] {base}: {n} pairs in {t_data:.1f} s, trained in {t_train:.1f} s; held-out "
-              f"per-pair max|err| {err_net:.3e} A (base only {err_base:.3e}), rms {rms_net:.3e} "
-              f"(base only {rms_base:.3e})", flush=True)
-    # ---- validation of the sample's far shell (rule7 + network), then timed steps
-    near = B.Plan(ctx, c.loops, c.box, 1, B.PREC_FP32C)
-    sur = surs["rule7"]
+    t0 = time.perf_counter()
+    cs = C.CubatureSum(ctx, x, y, z, c.loops, c.box, img, k_exact, near_tol=near_tol)
+    t_setup = time.perf_counter() - t0
     u = [torch.zeros_like(x) for _ in range(3)]
-    grid_h = c.box[0] / 32          # the rule's far shell on an L/32 grid, interpolated
-    rep = S.far_shell(ctx, sur, x, y, z, u, c.loops, infos, c.box, img, tol=1e-5,
-                      exact_if_failed=False, baseline="rule7", grid_h=grid_h)
-    rep_d = S.far_shell(ctx, surs["dipole"], x, y, z, [torch.zeros_like(x) for _ in range(3)],
-                        c.loops, infos, c.box, img, tol=1e-5, exact_if_failed=False,
-                        baseline="dipole")
-    print(f"[c5] validation rule7+net: max|err| {rep.max_err:.3e} A (rule7 only "
-          f"{rep.base_only_err:.3e}) -> {'surrogate' if rep.used_surrogate else 'FALLBACK'}; "
-          f"dipole+net {rep_d.max_err:.3e} (dipole only {rep_d.base_only_err:.3e})", flush=True)
-    ts = []
+    for _ in range(max(1, args.warmup)):
+        cs.run(x, y, z, *u)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ms = []
     for k in range(args.steps):
-        for t in u:
-            t.zero_()
+        ev[0].record(stream)
+        cs.run(x, y, z, *u)
+        ev[1].record(stream)
         torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        near.run_torch(x, y, z, *u, accumulate=True)
-        if rep.used_surrogate:
-            S.base_far_shell_torch(ctx, x, y, z, c.loops, c.box, img, "rule7", u=u,
-                                   accumulate=True, grid_h=grid_h)
-            sur.apply_torch(x, y, z, *u, infos, c.box)
-        else:
-            fp = B.Plan(ctx, c.loops, c.box, img, B.PREC_FP32C)
-            S.set_inner(fp, 1)
-            fp.run_torch(x, y, z, *u, accumulate=True)
-        torch.cuda.synchronize()
-        ts.append(time.perf_counter() - t1)
-        print(f"[c5] step {k + 1}: {ts[-1] * 1e3:.1f} ms", flush=True)
-    t_sur = float(np.mean(ts))
-    # ---- the plain GPU sum over +-4, timed on a contiguous eighth of the atoms (the
-    # same spatial coherence per wave as the full run) and scaled
-    idx = torch.arange(0, c.n // 8, device=dev)
+        ms.append(ev[0].elapsed_time(ev[1]))
+        print(f"[c5] step {k + 1}: {ms[-1]:.1f} ms", flush=True)
+    t_step = float(np.mean(ms))
+    cs.near.profile(True)
+    cs.run(x, y, z, *u)
+    far_ms, near_ms = cs.near.kernel_ms()
+    torch.cuda.synchronize()
+    # accuracy against the FP64 plain sum
+    stride = 1 if getattr(args, "c5_all", False) else 8
+    idx = torch.arange(0, c.n, stride, device=dev)
     xs, ys, zs = x[idx].contiguous(), y[idx].contiguous(), z[idx].contiguous()
     v = [torch.zeros_like(xs) for _ in range(3)]
+    p64 = B.Plan(ctx, c.loops, c.box, img, B.PREC_FP64)
+    t0 = time.perf_counter()
+    p64.run_torch(xs, ys, zs, *v)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    p64.free()
+    err = max(float((a[idx] - b).abs().max()) for a, b in zip(u, v))
+    print(f"[c5] max|du| vs the FP64 plain sum on {idx.numel()} atoms (every {stride}th): "
+          f"{err:.3e} A ({t64:.1f} s of FP64)", flush=True)
+    del v
+    # the plain GPU sum (FP32C) on a contiguous eighth, scaled to the sample
+    m = c.n // 8
+    w = [torch.zeros(m, dtype=torch.float64, device=dev) for _ in range(3)]
     plain = B.Plan(ctx, c.loops, c.box, img, B.PREC_FP32C)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    plain.run_torch(xs, ys, zs, *v)
-    torch.cuda.synchronize()
-    t_plain = (time.perf_counter() - t1) * c.n / idx.numel()
-    err_total = max(float((a[idx] - b).abs().max()) for a, b in zip(u, v))
-    print(f"[c5] plain +-4 sum: {t_plain:.2f} s per sample (scaled from the first eighth of "
-          f"the atoms); this path {t_sur:.3f} s; max|du| vs plain (those atoms) "
-          f"{err_total:.3e} A", flush=True)
-    _emit({"metric": "ms per 4M-atom sample, images +-4 (far shell by surrogate)",
-           "value": t_sur * 1e3, "unit": "ms", "n_gpus": 1, "steps": args.steps,
-           "warmup": 0, "ms_per_step": t_sur * 1e3, "higher_is_better": False,
-           "scaling": "weak", "vs_baseline": None,
-           "dtype": "f32 line +-1; f32 loopon rule far shell on an L/32 grid; bf16 MLP "
-                    "(fp32 accumulate)",
+    plain.profile(True)
+    plain.run_torch(x[:m], y[:m], z[:m], *w)
+    pf, pn = plain.kernel_ms()
+    plain.free()
+    t_plain = (pf + pn) * c.n / m
+    print(f"[c5] plain +-4 FP32C sum: {t_plain / 1e3:.2f} s per sample (scaled from the first "
+          f"eighth); cubature form {t_step:.1f} ms ({t_plain / t_step:.0f}x)", flush=True)
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        sub = np.arange(0, c.n, c.n // 256)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle.line_sum(c.x[sub], c.y[sub], c.z[sub], c.loops, c.box, img)
+            times.append(time.perf_counter() - t0)
+        dt = float(np.median(times))
+        ev_cpu = sub.size * c.segments() * (2 * img + 1) ** 3
+        cpu = {"value": ev_cpu / dt, "unit": "evals/s", "cores": oracle.threads(), "kind": "port",
+               "sample": f"{sub.size} atoms spread over the sample, all 200 loops, images +-4, "
+                         f"plain sum; median of 3 runs, {dt:.1f} s each",
+               "s_per_sample_implied": c.n * c.segments() * (2 * img + 1) ** 3 / (ev_cpu / dt)}
+    plain_evals = c.n * c.segments() * (2 * img + 1) ** 3
+    _emit({"metric": "ms per 4M-atom sample, images +-4 (cubature form)",
+           "value": t_step, "unit": "ms", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": t_step, "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "max_abs_err_A": err,
+           "max_abs_err_atoms": int(idx.numel()),
+           "dtype": "f32 line and cubature, f64 image sums and near slab; f32 loopon "
+                    "cubature on grids",
            "data": "synthetic",
-           "config": {"workload": "FCC 100^3 (4M atoms), 200 loops, images +-4",
-                      "plain_gpu_ms": t_plain * 1e3, "speedup_vs_plain": t_plain / t_sur,
-                      "max_abs_err_vs_plain_A": err_total, "validation": rep.__dict__,
-                      "validation_dipole": rep_d.__dict__, "training": train,
-                      "plain_evals": c.n * c.segments() * (2 * img + 1) ** 3},
-           "roofline": None, "cpu_baseline": None})
+           "config": {"workload": "FCC 100^3 (4M atoms), 200 loops as the 4M-atom sample, "
+                                  "images +-4", "k_exact": k_exact, "near_tol_A": near_tol,
+                      "rule_loops": cs.eligible, "images_on_grid": cs.skipped,
+                      "setup_s": t_setup, "plain_gpu_ms": t_plain,
+                      "speedup_vs_plain": t_plain / t_step, "plain_evals": plain_evals,
+                      "plain_evals_per_s_equivalent": plain_evals / (t_step * 1e-3)},
+           "roofline": {"bound": "valu", "achieved": None, "peak": FP32_PEAK, "unit": "TFLOP/s",
+                        "frac": None, "traffic": None,
+                        "kernel": "line_kernel<PASS_FAR, LOOP_GLIDE, true> (cubature far pass)",
+                        "kernel_ms": far_ms, "near_kernel_ms": near_ms,
+                        "grids_ms": t_step - far_ms - near_ms},
+           "cpu_baseline": cpu})
+    cs.free()
     ctx.close()
 
 

@@ -156,6 +156,9 @@ def lib():
                                             _i, ctypes.POINTER(_vp)]
         L.disco_hip_plan_run.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]
         L.disco_hip_plan_run_noalloc.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]
+        L.disco_hip_plan_set_cubature.argtypes = [_vp, _d, _d, ctypes.POINTER(_i)]
+        L.disco_hip_line_cubature.argtypes = [ctypes.POINTER(_LineLoop), _i, _dp]
+        L.disco_hip_plan_set_image_skip.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint), _i]
         L.disco_hip_plan_profile.argtypes = [_vp, _i]
         L.disco_hip_plan_kernel_ms.argtypes = [_vp, _dp, _i]
         L.disco_hip_plan_evals_per_atom.argtypes = [_vp]
@@ -357,6 +360,16 @@ def _sum_host(ctx, kind, x, y, z, u, loops, box, img, prec, timing):
     return ux, uy, uz
 
 
+def line_cubature(loops) -> np.ndarray:
+    """Per loop: (eligible, centre weight, ring weight, ring radius, cut radius)
+    of the 7-point cubature the cubature form uses (host only)."""
+    arr, keep = line_structs(loops)
+    out = np.zeros((max(1, len(loops)), 5))
+    _check(lib().disco_hip_line_cubature(arr, len(loops), out.ctypes.data_as(_dp)))
+    del keep
+    return out[:len(loops)]
+
+
 def line_sum(ctx, x, y, z, loops, box, img=1, prec=PREC_FP64, u=None, timing=None):
     """Host-buffer line sum: u (+)= sum over loops x images (one launch)."""
     n = len(x)
@@ -390,6 +403,17 @@ class Plan:
                                                   float(box[1]), float(box[2]), int(img),
                                                   int(prec), ctypes.byref(self._h)))
         del keep
+
+    def set_cubature(self, k_exact: float, k_rule: float | None = None) -> int:
+        """Cubature form of the far pass (FP32C line plans): exact within k_exact
+        loop radii of a loop image, the 7-point cubature beyond k_rule (default
+        k_exact: a hard switch), a smooth blend between; k_exact <= 0 switches it
+        off.  Returns the number of loops that use it."""
+        k_rule = k_exact if k_rule is None else k_rule
+        n = _i(0)
+        self.ctx.check(lib().disco_hip_plan_set_cubature(self._h, float(k_exact), float(k_rule),
+                                                         ctypes.byref(n)))
+        return n.value
 
     def profile(self, on: bool = True):
         """Record HIP events around each kernel of later runs."""

@@ -153,11 +153,24 @@ enum { PASS_FP64 = 0, PASS_FAR = 1, PASS_NEAR = 2 };
 // fewer registers.
 enum { FP64_GLOBAL = 1, FP64_FRAME = 2 };
 
+// Images a line pass leaves out: the inner shell (max(|i|,|j|,|k|) <= inner)
+// and the loop's skipped images of [-1, 1]^3 (wave-uniform).
+__device__ __forceinline__ bool image_out(const LineLoop& L, int di, int dj, int dk, int inner)
+{
+    const int m = max(abs(di), max(abs(dj), abs(dk)));
+    if (m <= inner) return true;
+    return m <= 1 && ((L.img_skip >> ((di + 1) * 9 + (dj + 1) * 3 + (dk + 1))) & 1u);
+}
+
 // FP32 far pass of one flat loop: every lane evaluates every image (uniform
 // control flow keeps the segment table in scalar loads); near lanes discard
 // it.  Each image's field is turned to global axes in float and added to the
-// FP64 sums.
-template <int KIND>
+// FP64 sums.  CUBATURE: for a loop whose 9-point cubature matches its cut
+// (rule_ok), each lane weighs the exact field by 1 - w and the cubature's by w,
+// w = rule_window(|q|^2) (0 near the loop, 1 far from it); a wave evaluates the
+// exact field only when one of its lanes (outside the near slab) has w < 1,
+// and the cubature only when one has w > 0.
+template <int KIND, bool CUBATURE>
 __device__ __forceinline__ void far_flat_loop(const FrameAtom& f, const LineLoop& L,
                                               const SegP* __restrict__ s, int img, int inner,
                                               double& sx, double& sy, double& sz)
@@ -165,12 +178,31 @@ __device__ __forceinline__ void far_flat_loop(const FrameAtom& f, const LineLoop
     for (int di = -img; di <= img; di++)
         for (int dj = -img; dj <= img; dj++)
             for (int dk = -img; dk <= img; dk++) {
-                if (max(abs(di), max(abs(dj), abs(dk))) <= inner) continue;   // inner shell
+                if (image_out(L, di, dj, dk, inner)) continue;
                 double q1, q2, q3;
                 const bool near = frame_image(f, di, dj, dk, L, q1, q2, q3);
-                float fo, o1, o2, o3;
-                line_image_flat_f32<DevMath, KIND>((float)q1, (float)q2, (float)q3, s, L.nv, fo,
-                                                   o1, o2, o3);
+                const float g1 = (float)q1, g2 = (float)q2, g3 = (float)q3;
+                float fo = 0.f, o1 = 0.f, o2 = 0.f, o3 = 0.f;
+                if (!CUBATURE || !L.rule_ok) {
+                    line_image_flat_f32<DevMath, KIND>(g1, g2, g3, s, L.nv, fo, o1, o2, o3);
+                } else {
+                    const float t = g1 * g1 + (g2 * g2 + g3 * g3);
+                    const float w = rule_window(t, L);
+                    if (DevMath::any(!near && w < 1.0f))
+                        line_image_flat_f32<DevMath, KIND>(g1, g2, g3, s, L.nv, fo, o1, o2, o3);
+                    if (DevMath::any(w > 0.0f)) {
+                        float c1, c2, c3;
+                        rule9_image_f32<DevMath>(g1, g2, g3, t, L, c1, c2, c3);
+                        // selects, so that a field not evaluated (or not finite
+                        // where its weight is zero) never enters the sum
+                        const float we = 1.0f - w;
+                        const bool all_c = w >= 1.0f, mix = w > 0.0f && !all_c;
+                        fo = all_c ? 0.0f : (mix ? we * fo : fo);
+                        o1 = all_c ? c1 : (mix ? we * o1 + w * c1 : o1);
+                        o2 = all_c ? c2 : (mix ? we * o2 + w * c2 : o2);
+                        o3 = all_c ? c3 : (mix ? we * o3 + w * c3 : o3);
+                    }
+                }
                 if (!near) {
                     sx += (double)(L.bf[0] * fo + (o1 * L.e1f[0] + o2 * L.e2f[0] + o3 * L.nf[0]));
                     sy += (double)(L.bf[1] * fo + (o1 * L.e1f[1] + o2 * L.e2f[1] + o3 * L.nf[1]));
@@ -181,7 +213,7 @@ __device__ __forceinline__ void far_flat_loop(const FrameAtom& f, const LineLoop
 
 // PASS_FAR sums loops of one kind (KIND), each kind in its own kernel so that
 // each is compiled (and its registers allocated) for its own form.
-template <int PASS, int KIND = LOOP_3D>
+template <int PASS, int KIND = LOOP_3D, bool CUBATURE = false>
 __device__ __forceinline__ bool atom_line(double px, double py, double pz,
                                           const LineLoop* __restrict__ loops, int l0, int l1,
                                           const SegF* __restrict__ segf,
@@ -200,7 +232,7 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
             for (int di = -img; di <= img; di++)
                 for (int dj = -img; dj <= img; dj++)
                     for (int dk = -img; dk <= img; dk++) {
-                        if (max(abs(di), max(abs(dj), abs(dk))) <= inner) continue;
+                        if (image_out(L, di, dj, dk, inner)) continue;
                         if (L.flat64) {
                             double q1, q2, q3;
                             frame_image(f, di, dj, dk, L, q1, q2, q3);
@@ -218,7 +250,7 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
         frame_atom(pcx, pcy, pcz, L, f);
         if (PASS == PASS_FAR && KIND != LOOP_3D) {
             const SegP* sp = reinterpret_cast<const SegP*>(segf + L.seg0);
-            far_flat_loop<KIND>(f, L, sp, img, inner, sx, sy, sz);
+            far_flat_loop<KIND, CUBATURE>(f, L, sp, img, inner, sx, sy, sz);
             continue;
         }
         if (PASS == PASS_NEAR) {
@@ -230,7 +262,7 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
             for (int di = i0; di <= i1; di++)
                 for (int dj = j0; dj <= j1; dj++)
                     for (int dk = k0; dk <= k1; dk++) {
-                        if (max(abs(di), max(abs(dj), abs(dk))) <= inner) continue;
+                        if (image_out(L, di, dj, dk, inner)) continue;
                         double q1, q2, q3;
                         if (!frame_image(f, di, dj, dk, L, q1, q2, q3)) continue;
                         any = true;
@@ -249,7 +281,7 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
         for (int di = -img; di <= img; di++)
             for (int dj = -img; dj <= img; dj++)
                 for (int dk = -img; dk <= img; dk++) {
-                    if (max(abs(di), max(abs(dj), abs(dk))) <= inner) continue;   // inner shell
+                    if (image_out(L, di, dj, dk, inner)) continue;
                     double q1, q2, q3;
                     const bool near = frame_image(f, di, dj, dk, L, q1, q2, q3);
                     const double qx = pcx - di * Lx, qy = pcy - dj * Ly, qz = pcz - dk * Lz;
@@ -376,7 +408,7 @@ __device__ __forceinline__ void atom_loopon(double px, double py, double pz,
 // (PASS_FAR): write u (or add).  At least 4 waves per SIMD: the FP64
 // instantiations fit 128 registers (from 132, two values spilled outside the
 // segment walk) and run ~2 % faster; the FP32 ones use 78 and are unaffected.
-template <int PASS, int KIND>
+template <int PASS, int KIND, bool CUBATURE = false>
 __global__ __launch_bounds__(kBlock, 4) void line_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
@@ -387,8 +419,8 @@ __global__ __launch_bounds__(kBlock, 4) void line_kernel(
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    atom_line<PASS, KIND>(x[i], y[i], z[i], loops, 0, nloops, segf, segd, Lx, Ly, Lz, img, inner,
-                          sx, sy, sz);
+    atom_line<PASS, KIND, CUBATURE>(x[i], y[i], z[i], loops, 0, nloops, segf, segd, Lx, Ly, Lz,
+                                    img, inner, sx, sy, sz);
     if (accumulate) { sx += ux[i]; sy += uy[i]; sz += uz[i]; }
     ux[i] = sx; uy[i] = sy; uz[i] = sz;
 }
@@ -660,6 +692,8 @@ struct disco_plan {
     double L[3] = {0, 0, 0};
     double work = 0;       // segments (or loopons) per image
     int kofs[5] = {0, 0, 0, 0, 0};   // line plans: loops of kind k are [kofs[k], kofs[k+1])
+    bool cubature = false;           // FP32C line plans: the far pass weighs in the cubature
+    std::vector<LineLoop> hloops;    // line plans: host copy of the packed loops
     int split = 1;                   // loopon plans: loop groups per launch (few atoms)
     double* d_split = nullptr;       // their partial sums (3 n split doubles, grow-only)
     size_t cap_split = 0;
@@ -791,10 +825,14 @@ int launch(disco_plan* p, int n, const double* x, const double* y, const double*
             for (int k = 0; k < 4; k++) {
                 const int l0 = p->kofs[k], cnt = p->kofs[k + 1] - l0;
                 if (cnt == 0) continue;
+                const bool cub = p->cubature;
                 auto kern = k == LOOP_3D ? line_kernel<PASS_FAR, LOOP_3D>
-                          : k == LOOP_FLAT ? line_kernel<PASS_FAR, LOOP_FLAT>
-                          : k == LOOP_GLIDE ? line_kernel<PASS_FAR, LOOP_GLIDE>
-                          : line_kernel<PASS_FAR, LOOP_PRISM>;
+                          : k == LOOP_FLAT ? (cub ? line_kernel<PASS_FAR, LOOP_FLAT, true>
+                                                  : line_kernel<PASS_FAR, LOOP_FLAT>)
+                          : k == LOOP_GLIDE ? (cub ? line_kernel<PASS_FAR, LOOP_GLIDE, true>
+                                                   : line_kernel<PASS_FAR, LOOP_GLIDE>)
+                          : (cub ? line_kernel<PASS_FAR, LOOP_PRISM, true>
+                                 : line_kernel<PASS_FAR, LOOP_PRISM>);
                 hipLaunchKernelGGL(kern, grid, block, 0, s, x, y, z, ux, uy, uz, n, lp + l0, cnt,
                                    sf, sd, p->L[0], p->L[1], p->L[2], p->img, p->inner, acc);
                 acc = 1;
@@ -862,6 +900,7 @@ int make_line_plan(disco_ctx* ctx, const disco_line_loop* loops, int nloops, dou
     p->L[0] = Lx; p->L[1] = Ly; p->L[2] = Lz;
     p->nloops = (int)pk.loops.size();
     p->work = pk.segments;
+    p->hloops = pk.loops;
     std::memcpy(p->kofs, pk.kofs, sizeof(p->kofs));
     const int rc = upload_plan(ctx, p, pk.loops.data(), pk.loops.size() * sizeof(LineLoop),
                                pk.segf.data(), pk.segf.size() * sizeof(SegF), pk.segd.data(),
@@ -1377,6 +1416,82 @@ int disco_hip_plan_set_inner(disco_plan* plan, int inner)
     if (!plan) return DISCO_ERR_ARG;
     if (inner < -1 || inner > plan->img) return fail(plan->ctx, DISCO_ERR_ARG, "inner shell out of range");
     plan->inner = inner;
+    return DISCO_OK;
+}
+
+int disco_hip_plan_set_cubature(disco_plan* plan, double k_exact, double k_rule, int* eligible)
+{
+    if (!plan) return DISCO_ERR_ARG;
+    disco_ctx* ctx = plan->ctx;
+    if (plan->kind != 0 || plan->prec != DISCO_PREC_FP32C)
+        return fail(ctx, DISCO_ERR_ARG, "the cubature form is for FP32C line plans");
+    if (k_exact <= 0.0) {
+        plan->cubature = false;
+        if (eligible) *eligible = 0;
+        return DISCO_OK;
+    }
+    if (!(k_exact >= 2.0) || !(k_rule >= k_exact) || !std::isfinite(k_rule))
+        return fail(ctx, DISCO_ERR_ARG, "cubature radii: need 2 <= k_exact <= k_rule < inf");
+    int ok = 0;
+    for (LineLoop& L : plan->hloops) {
+        if (!L.rule_ok) continue;
+        ok++;
+        const double R = L.rR;
+        // no lane of the near slab (|h| < hn, in-plane radius^2 < rn2) is weighted
+        const double t0 = std::fmax(k_exact * k_exact * R * R, 1.0001 * (L.rn2 + L.hn * L.hn));
+        const double t1 = std::fmax(k_rule * k_rule * R * R, t0);
+        L.rt0 = (float)t0;
+        L.rwinv = t1 > t0 * (1.0 + 1e-6) ? (float)(1.0 / (t1 - t0)) : 1e30f;
+    }
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (!plan->hloops.empty()) {
+        DISCO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        DISCO_HIP_TRY(ctx, hipMemcpy(plan->d_loops, plan->hloops.data(),
+                                     plan->hloops.size() * sizeof(LineLoop),
+                                     hipMemcpyHostToDevice));
+    }
+    plan->cubature = true;
+    if (eligible) *eligible = ok;
+    return DISCO_OK;
+}
+
+int disco_hip_plan_set_image_skip(disco_plan* plan, const unsigned* skip, int nloops)
+{
+    if (!plan) return DISCO_ERR_ARG;
+    disco_ctx* ctx = plan->ctx;
+    if (plan->kind != 0) return fail(ctx, DISCO_ERR_ARG, "image skips are for line plans");
+    if (skip && nloops < 0) return fail(ctx, DISCO_ERR_ARG, "negative loop count");
+    if (skip)
+        for (const LineLoop& L : plan->hloops)
+            if (L.src >= nloops) return fail(ctx, DISCO_ERR_ARG, "skip array shorter than the loops");
+    for (LineLoop& L : plan->hloops) L.img_skip = skip ? skip[L.src] & ((1u << 27) - 1u) : 0u;
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (!plan->hloops.empty()) {
+        DISCO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        DISCO_HIP_TRY(ctx, hipMemcpy(plan->d_loops, plan->hloops.data(),
+                                     plan->hloops.size() * sizeof(LineLoop),
+                                     hipMemcpyHostToDevice));
+    }
+    return DISCO_OK;
+}
+
+int disco_hip_line_cubature(const disco_line_loop* loops, int nloops, double* out)
+{
+    if (nloops < 0 || (nloops > 0 && (!loops || !out))) return DISCO_ERR_ARG;
+    for (int l = 0; l < nloops; l++) {
+        LinePack pk;
+        const std::string e = pack_line(loops + l, 1, 1.0, 1.0, 1.0, pk);
+        if (!e.empty()) return DISCO_ERR_ARG;
+        double* o = out + 5 * (size_t)l;
+        o[0] = o[1] = o[2] = o[3] = o[4] = 0.0;
+        if (pk.loops.empty()) continue;
+        const LineLoop& L = pk.loops[0];
+        o[0] = L.rule_ok;
+        o[1] = L.rw0;
+        o[2] = L.rw1;
+        o[3] = L.rrho;
+        o[4] = L.rR;
+    }
     return DISCO_OK;
 }
 

@@ -85,9 +85,25 @@ struct alignas(64) LineLoop {
     int flat64;             // flat to double resolution: the FP64 table holds SegPD
     double reach;           // > sqrt(rn2 + hn^2): a near image has its apex this close
     double iLx, iLy, iLz;   // 1 / box lengths (0 for a zero length)
-    double pad;
+    // 9-point cubature of the flat cut (the far-image form of a cubature plan):
+    // weights (areas) of the centre and of the eight ring points at angles
+    // j pi/4 from e1 and radius rrho
+    float rw0, rw1, rrho;
+    float rmp[4];           // rm . P for the ring directions (1, 0), (0, 1), (1, 1)/sqrt2,
+                            // (-1, 1)/sqrt2 (times rrho)
+    float rm[3];            // c2 m in the frame, c2 = -3 k1 per unit area
+    float rcb;              // c1 (m.n), c1 = -k1 (1 - 2 nu) per unit area
+    float rmf[3];           // m = b / |b| in the frame
+    float rbm;              // |b|
+    float rt0, rwinv;       // window: w(r^2) = s(clamp((r^2 - rt0) rwinv, 0, 1)), s quintic
+    int rule_ok;            // the cubature matches the cut's moments through degree 5
+    float rR;               // the cut's radius: largest in-plane vertex distance from c
+    int src;                // index of the loop in the caller's loop array
+    unsigned img_skip;      // images (i, j, k) in [-1, 1]^3 left out of the sum: bit
+                            // 9 (i + 1) + 3 (j + 1) + (k + 1) (the caller sums them)
+    int rpad[11];
 };
-static_assert(sizeof(LineLoop) == 320, "loop record size");
+static_assert(sizeof(LineLoop) == 448, "loop record size");
 
 // One loop of a loopon plan; loopons are stored relative to (cx, cy, cz).
 struct alignas(128) LooponLoop {
@@ -441,6 +457,81 @@ DISCO_HD void line_image_flat_f32(float q1, float q2, float h, const SegP* __res
     o1 = (float)kCJ * w.sw1 - (float)kC3 * x1;
     o2 = (float)kCJ * w.sw2 - (float)kC3 * x2;
     o3 = (float)kCJ * w.sw3 - (float)kC3 * x3;
+}
+
+// ------------------------------------------------------------ 9-point cubature, FP32
+// Far field of a flat loop image from the Kroupa term (Code/kroupa.hpp:52-73)
+// integrated over its cut by a 9-point rule: the centre (area rw0) and eight
+// ring points P_j (area rw1 each) at radius rho, angles j pi/4, all in the
+// loop's plane.  The lane sits at q = (q1, q2, h) in the loop frame; per point
+// rho_p = q - P and, with unit-area constants,
+//   t1 += rho^-3 (c2 m.rho),  s2 += rho^-3,
+//   co  = rho^-3 (h (c2 m.rho) rho^-2 - c1 m.n),  (a1, a2, ac) += co (r1, r2, 1)
+// (loopon_flat_f32's sums); the field in the frame is
+//   o = |b| [ (0, 0, (c1/c2) t1) + m (c1 h s2) + (a1, a2, h ac) ].
+// The ring points come in pairs +-P: |q -+ P|^2 = (|q|^2 + rho^2) -+ 2 q.P and
+// c2 m.(q -+ P) = c2 m.q -+ rmp, so a pair shares its dot products, and
+// sum co (r1, r2) = (q1, q2) sum co - sum co P: per pair only (co+ - co-) P.
+template <class M>
+DISCO_HD void rule_ring_pair(float base, float d, float mq, float mp, float h, float cb,
+                             float& t1, float& s2, float& ac, float& dc)
+{
+    const float rp = base - d, rn = base + d;          // |q - P|^2, |q + P|^2
+    const float ip = M::rsqrt_f(rp), in = M::rsqrt_f(rn);
+    const float ip2 = ip * ip, in2 = in * in;
+    const float ip3 = ip2 * ip, in3 = in2 * in;
+    const float mrp = mq - mp, mrn = mq + mp;
+    t1 += ip3 * mrp + in3 * mrn;
+    s2 += ip3 + in3;
+    const float cp = ip3 * ((h * mrp) * ip2 - cb);
+    const float cn = in3 * ((h * mrn) * in2 - cb);
+    ac += cp + cn;
+    dc = cp - cn;
+}
+
+template <class M>
+DISCO_HD void rule9_image_f32(float q1, float q2, float h, float t, const LineLoop& L,
+                              float& o1, float& o2, float& o3)
+{
+    constexpr float kC1 = (float)(-kC3 * (1.0 - 2.0 * kNu));
+    constexpr float kR12 = (float)((1.0 - 2.0 * kNu) / 3.0);     // c1 / c2
+    constexpr float kS = 0.70710678f;                            // 1 / sqrt 2
+    const float mq = L.rm[0] * q1 + (L.rm[1] * q2 + L.rm[2] * h);
+    const float cb = L.rcb;
+    // centre: rho = q, |q|^2 = t
+    const float i0 = M::rsqrt_f(t);
+    const float i02 = i0 * i0, i03 = i02 * i0;
+    const float c0 = i03 * ((h * mq) * i02 - cb);
+    // ring
+    const float rho = L.rrho;
+    const float base = t + rho * rho;
+    const float u1 = (2.0f * rho) * q1, u2 = (2.0f * rho) * q2;   // 2 q.P on the axes
+    const float v1 = (u1 + u2) * kS, v2 = (u2 - u1) * kS;          // and on the diagonals
+    float t1 = 0.f, s2 = 0.f, ac = 0.f, d0, d1, d2, d3;
+    rule_ring_pair<M>(base, u1, mq, L.rmp[0], h, cb, t1, s2, ac, d0);
+    rule_ring_pair<M>(base, u2, mq, L.rmp[1], h, cb, t1, s2, ac, d1);
+    rule_ring_pair<M>(base, v1, mq, L.rmp[2], h, cb, t1, s2, ac, d2);
+    rule_ring_pair<M>(base, v2, mq, L.rmp[3], h, cb, t1, s2, ac, d3);
+    // sum over the ring of co P: pairs along (1,0), (0,1), (1,1)/sqrt2, (-1,1)/sqrt2
+    const float A1 = rho * (d0 + (d2 - d3) * kS);
+    const float A2 = rho * (d1 + (d2 + d3) * kS);
+    const float w0 = L.rw0, w1 = L.rw1;
+    const float T1 = w0 * (i03 * mq) + w1 * t1;
+    const float S2 = kC1 * (h * (w0 * i03 + w1 * s2));
+    const float AC = w0 * c0 + w1 * ac;
+    o1 = L.rbm * (L.rmf[0] * S2 + (q1 * AC - w1 * A1));
+    o2 = L.rbm * (L.rmf[1] * S2 + (q2 * AC - w1 * A2));
+    o3 = L.rbm * ((kR12 * T1 + L.rmf[2] * S2) + h * AC);
+}
+
+// Window weight of the cubature for a lane at squared distance t from the
+// loop centre: 0 inside rt0, 1 beyond rt0 + 1/rwinv, the quintic smoothstep
+// x^3 (10 - 15 x + 6 x^2) between (a hard switch when rwinv is huge).
+DISCO_HD float rule_window(float t, const LineLoop& L)
+{
+    float x = (t - L.rt0) * L.rwinv;
+    x = x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
+    return (x * x) * x * (x * (x * 6.0f - 15.0f) + 10.0f);
 }
 
 // ------------------------------------------------------------ line, FP64

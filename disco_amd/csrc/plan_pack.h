@@ -44,6 +44,101 @@ struct LooponPack {
 // FP64 in the FP32C mode.
 inline double near_height(double bmag) { return std::fmax(3.0 * bmag, 6.0); }
 
+// The 9-point cubature of a flat cut (the fields rw0 ... rR of LineLoop):
+// p1, p2 are the vertices relative to the apex in the loop frame (e1 along
+// the farthest vertex).  Its centre weight w0 and eight ring points of weight
+// w1 at radius rho (angles j pi/4 from e1) match the cut's area A,
+// I2 = int r^2 dA and I4 = int r^4 dA: w0 + 8 w1 = A, 8 w1 rho^2 = I2,
+// 8 w1 rho^4 = I4 (for a disc: rho^2 = 2R^2/3, w0 = A/4, w1 = 3A/32).  The
+// ring of eight is isotropic through rank 7, so the rule's first angular
+// error is at degree 8 and its degree-6 error is the radial I6 alone.  The
+// rule is used (rule_ok) only if it also matches every other moment
+// int x^a y^b dA, a + b <= 5, of the cut to 1e-7 A R^(a+b): then the far
+// field's error is O((R/r)^6) of the field (cuts with moments isotropic
+// through rank 5 -- circles, regular polygons of six sides or more).  The
+// cut's moments come from its fan triangles (c, V[k-1], V[k]) with Radon's
+// 7-point rule on a triangle, which is exact for polynomials of degree 5.
+inline void pack_rule9(const std::vector<double>& p1, const std::vector<double>& p2,
+                       const double b[3], const double e1[3], const double e2[3],
+                       const double n[3], double rmax, LineLoop& L)
+{
+    const int nv = (int)p1.size();
+    double M[6][6] = {};   // M[a][b] = int x^a y^b dA, a + b <= 5
+    const double s15 = std::sqrt(15.0);
+    const double ta = (6.0 - s15) / 21.0, tb = (9.0 + 2.0 * s15) / 21.0;
+    const double tc = (6.0 + s15) / 21.0, td = (9.0 - 2.0 * s15) / 21.0;
+    const double wa = (155.0 - s15) / 1200.0, wc = (155.0 + s15) / 1200.0;
+    const double bary[7][3] = {{1.0 / 3, 1.0 / 3, 1.0 / 3}, {ta, ta, tb}, {ta, tb, ta},
+                               {tb, ta, ta}, {tc, tc, td}, {tc, td, tc}, {td, tc, tc}};
+    const double bw[7] = {9.0 / 40.0, wa, wa, wa, wc, wc, wc};
+    for (int k = 0; k < nv; k++) {
+        const int j = (k + nv - 1) % nv;
+        const double area = 0.5 * (p1[j] * p2[k] - p2[j] * p1[k]);   // signed, apex at 0
+        for (int q = 0; q < 7; q++) {
+            const double x = bary[q][1] * p1[j] + bary[q][2] * p1[k];
+            const double y = bary[q][1] * p2[j] + bary[q][2] * p2[k];
+            double xa = 1.0;
+            for (int a = 0; a <= 5; a++) {
+                double yb = 1.0;
+                for (int c = 0; a + c <= 5; c++) {
+                    M[a][c] += area * bw[q] * xa * yb;
+                    yb *= y;
+                }
+                xa *= x;
+            }
+        }
+    }
+    const double A = M[0][0], I2 = M[2][0] + M[0][2], I4 = M[4][0] + 2.0 * M[2][2] + M[0][4];
+    L.rule_ok = 0;
+    L.rR = (float)rmax;
+    if (!(A > 0.0 && I2 > 0.0 && I4 > 0.0 && rmax > 0.0)) return;
+    const double rho = std::sqrt(I4 / I2), w1 = I2 * I2 / (8.0 * I4), w0 = A - 8.0 * w1;
+    if (!(w0 >= 0.0)) return;
+    // the rule's own moments
+    double R[6][6] = {};
+    R[0][0] = w0;
+    for (int j = 0; j < 8; j++) {
+        const double th = j * (kPi / 4.0);
+        const double x = rho * std::cos(th), y = rho * std::sin(th);
+        double xa = 1.0;
+        for (int a = 0; a <= 5; a++) {
+            double yb = 1.0;
+            for (int c = 0; a + c <= 5; c++) {
+                R[a][c] += w1 * xa * yb;
+                yb *= y;
+            }
+            xa *= x;
+        }
+    }
+    double worst = 0.0, rp = 1.0;
+    for (int d = 0; d <= 5; d++, rp *= rmax)
+        for (int a = 0; a <= d; a++)
+            worst = std::fmax(worst, std::fabs(M[a][d - a] - R[a][d - a]) / (A * rp));
+    const double bmag = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    if (!(worst <= 1e-7) || !(bmag > 0.0)) return;
+    const double mf[3] = {(b[0] * e1[0] + b[1] * e1[1] + b[2] * e1[2]) / bmag,
+                          (b[0] * e2[0] + b[1] * e2[1] + b[2] * e2[2]) / bmag,
+                          (b[0] * n[0] + b[1] * n[1] + b[2] * n[2]) / bmag};
+    const double c1 = -kC3 * (1.0 - 2.0 * kNu), c2 = -3.0 * kC3;   // per unit area
+    L.rw0 = (float)w0;
+    L.rw1 = (float)w1;
+    L.rrho = (float)rho;
+    for (int i = 0; i < 3; i++) {
+        L.rm[i] = (float)(c2 * mf[i]);
+        L.rmf[i] = (float)mf[i];
+    }
+    const double s2 = std::sqrt(0.5);
+    L.rmp[0] = (float)(c2 * mf[0] * rho);
+    L.rmp[1] = (float)(c2 * mf[1] * rho);
+    L.rmp[2] = (float)(c2 * (mf[0] + mf[1]) * s2 * rho);
+    L.rmp[3] = (float)(c2 * (mf[1] - mf[0]) * s2 * rho);
+    L.rcb = (float)(c1 * mf[2]);
+    L.rbm = (float)bmag;
+    L.rt0 = 0.0f;
+    L.rwinv = 0.0f;
+    L.rule_ok = 1;
+}
+
 // Returns "" on success, else the reason the input is rejected.
 inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, double Ly,
                              double Lz, LinePack& out)
@@ -101,6 +196,7 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, d
         L.rn2 = (rmax + H) * (rmax + H);
         L.seg0 = (int)out.segd.size();
         L.nv = nv;
+        L.src = l;
 
         // loop frame: e1 along the in-plane offset of the vertex farthest from
         // the apex, e2 = n x e1
@@ -166,6 +262,10 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, d
             L.e1f[i] = (float)e1v[i];
             L.e2f[i] = (float)e2v[i];
             L.nf[i] = (float)nv3[i];
+        }
+        if (L.kind != LOOP_3D) {
+            const double n3[3] = {nx, ny, nz};
+            pack_rule9(p1, p2, lp.b, e1v, e2v, n3, rmax, L);
         }
         const double fb3 = L.kind == LOOP_GLIDE ? 0.0 : b3;
         const double fb1 = L.kind == LOOP_PRISM ? 0.0 : b1;

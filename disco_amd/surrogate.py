@@ -168,14 +168,16 @@ def rule7_loopons(lp: B.LoopData):
     return rule7_loopons_many([lp])
 
 
-def rule7_loopons_many(loops):
+def rule7_loopons_many(loops, ring=6):
     """rule7_loopons of many loops, computed together for the loops of each
-    vertex count: [centre, ring] per loop, in the loops' order."""
+    vertex count: [centre, ring] per loop, in the loops' order.  ring = 8 gives
+    the 9-point rule (w0 + 8 w1 = A, 8 w1 rho^2 = I2, 8 w1 rho^4 = I4; ring
+    angles j pi/4), whose first angular error is at degree 8 instead of 6."""
     out = [None] * (2 * len(loops))
     by_nv = {}
     for i, lp in enumerate(loops):
         by_nv.setdefault(len(lp.vx), []).append(i)
-    th = 2.0 * np.pi * np.arange(6) / 6.0
+    th = 2.0 * np.pi * np.arange(ring) / ring
     cs, sn = np.cos(th), np.sin(th)
     for nv, idx in by_nv.items():
         C = np.array([[loops[i].cx, loops[i].cy, loops[i].cz] for i in idx])        # (L, 3)
@@ -196,16 +198,16 @@ def rule7_loopons_many(loops):
         I2 = np.einsum("lj,q,lqj->l", ar, _DW, r2)
         I4 = np.einsum("lj,q,lqj->l", ar, _DW, r2 * r2)
         rho = np.sqrt(I4 / I2)
-        w1 = I2 * I2 / (6.0 * I4)
-        w0 = A - 6.0 * w1
+        w1 = I2 * I2 / (ring * I4)
+        w0 = A - ring * w1
         V0 = V[:, 0]
         e1 = V0 - (V0 * n).sum(1)[:, None] * n
         e1 /= np.sqrt((e1 * e1).sum(1))[:, None]
         e2 = np.stack([n[:, 1] * e1[:, 2] - n[:, 2] * e1[:, 1],
                        n[:, 2] * e1[:, 0] - n[:, 0] * e1[:, 2],
                        n[:, 0] * e1[:, 1] - n[:, 1] * e1[:, 0]], 1)
-        ring = C[:, None] + rho[:, None, None] * (cs[None, :, None] * e1[:, None] +
-                                                 sn[None, :, None] * e2[:, None])     # (L, 6, 3)
+        pts_r = C[:, None] + rho[:, None, None] * (cs[None, :, None] * e1[:, None] +
+                                                  sn[None, :, None] * e2[:, None])    # (L, ring, 3)
         for k, i in enumerate(idx):
             b = np.array(loops[i].burgers())
             bm = float(np.linalg.norm(b))
@@ -214,8 +216,8 @@ def rule7_loopons_many(loops):
             out[2 * i] = B.LoopData(phys=(bm, bm, bm, *m, *nk, float(w0[k])), line=False,
                                     x=C[k, :1].copy(), y=C[k, 1:2].copy(), z=C[k, 2:3].copy())
             out[2 * i + 1] = B.LoopData(phys=(bm, bm, bm, *m, *nk, float(w1[k])), line=False,
-                                        x=ring[k, :, 0].copy(), y=ring[k, :, 1].copy(),
-                                        z=ring[k, :, 2].copy())
+                                        x=pts_r[k, :, 0].copy(), y=pts_r[k, :, 1].copy(),
+                                        z=pts_r[k, :, 2].copy())
     return out
 
 
@@ -224,6 +226,8 @@ def base_loopons(loops, kind: str):
         return [dipole_loopon(l) for l in loops]
     if kind == "rule7":
         return rule7_loopons_many(loops)
+    if kind == "rule9":
+        return rule7_loopons_many(loops, ring=8)
     raise ValueError(f"unknown baseline {kind!r}")
 
 

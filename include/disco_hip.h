@@ -167,6 +167,28 @@ int disco_hip_plan_kernel_ms(disco_plan *plan, double *ms, int nms);
  * are skipped (inner = -1, the default, keeps them all; inner <= img).  A
  * far-shell plan (inner = 1) is what the surrogate replaces and validates. */
 int disco_hip_plan_set_inner(disco_plan *plan, int inner);
+/* (No counterpart in the reference.)  Cubature form of an FP32C line plan's
+ * far pass, for sums over many images: for each loop whose cut a 9-point
+ * cubature of the Kroupa term represents (the centre and a ring of eight
+ * matching the cut's area, second and fourth radial moments and every moment
+ * through degree 5 -- circles and regular polygons of >= 6 sides), a lane at
+ * distance r from a loop image's centre takes the exact field for
+ * r < k_exact R, the cubature for r > k_rule R and a smooth blend between
+ * (R = the cut's radius; the near slab is always exact).  The cubature's error
+ * falls as (R/r)^8: for |b| = 2.56 Angstrom about 0.15 (R/r)^8 Angstrom at
+ * most.  k_exact <= 0 switches it off (the default); otherwise
+ * 2 <= k_exact <= k_rule.  eligible (may be NULL) receives the number of loops
+ * that use it; the others stay exact.  Synchronous. */
+int disco_hip_plan_set_cubature(disco_plan *plan, double k_exact, double k_rule, int *eligible);
+/* (No counterpart in the reference.)  Leave images of the +-1 shell out of a
+ * line plan's sums: bit 9 (i + 1) + 3 (j + 1) + (k + 1) of skip[l] drops image
+ * (i, j, k) of input loop l from every pass (the caller sums it another way,
+ * say as a cubature on a grid when it lies far from every atom).  skip = NULL
+ * clears every mask.  Synchronous. */
+int disco_hip_plan_set_image_skip(disco_plan *plan, const unsigned *skip, int nloops);
+/* Host only: for each loop, {eligible (0/1), centre weight, ring weight, ring
+ * radius, cut radius} of its 9-point cubature (out: 5 doubles per loop). */
+int disco_hip_line_cubature(const disco_line_loop *loops, int nloops, double *out);
 /* (No counterpart.) A loopon plan evaluated at few atoms (say the nodes of a
  * coarse grid) splits its loops into `groups` contiguous groups summed by
  * separate workgroups of one launch, then added in group order (1 = off,

@@ -1,0 +1,179 @@
+"""The cubature form of wide image sums on the GPU, against the FP64 plain sum
+(itself pinned to the oracle by test_gpu_line.py): the 4M-atom sample at +-2 on
+every atom, the 200-loop +-4 sample (images +-4, 729 per loop) at random atoms
+and atoms next to the loop cuts, the image skips, loop forms the rule does not
+represent, and argument errors."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from loop_kinds import BOX, atoms_near, kind_loops
+from disco_amd import backend as B
+from disco_amd import cubature as C
+from disco_amd import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5          # max |du| against the FP64 plain sum, Angstrom
+K_EXACT, NEAR_TOL = 5.0, 1e-7
+
+
+def _dev(*arrs):
+    import torch
+    return [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in arrs]
+
+
+def _plain64(ctx, x, y, z, loops, box, img):
+    import torch
+    u = [torch.zeros_like(x) for _ in range(3)]
+    p = B.Plan(ctx, loops, box, img, B.PREC_FP64)
+    p.run_torch(x, y, z, *u)
+    torch.cuda.synchronize()
+    p.free()
+    return u
+
+
+def _err(a, b):
+    return max(float((s - t).abs().max()) for s, t in zip(a, b))
+
+
+def test_c2_every_atom(hip_ctx):
+    """4,000,000 atoms, 50 loops, images +-2: the cubature form within 1e-5 A of
+    the FP64 plain sum on every atom."""
+    s = G.config_c2()
+    x, y, z = _dev(s.x, s.y, s.z)
+    ref = _plain64(hip_ctx, x, y, z, s.loops, s.box, s.img)
+    u = C.cubature_sum_torch(hip_ctx, x, y, z, s.loops, s.box, s.img, K_EXACT, near_tol=NEAR_TOL)
+    e = _err(u, ref)
+    print(f"C2 cubature vs FP64 plain, all {s.n} atoms: {e:.3e} A")
+    assert e < TOL
+
+
+def _near_cut_atoms(loops, rng, per_loop, dist):
+    pts = []
+    for lp in loops:
+        c = np.array([lp.cx, lp.cy, lp.cz])
+        n = np.array(lp.phys[6:9])
+        V = np.stack([lp.vx, lp.vy, lp.vz], 1)
+        R = np.linalg.norm(V - c, axis=1).max()
+        e1 = (V[0] - c) / np.linalg.norm(V[0] - c)
+        e2 = np.cross(n, e1)
+        r = rng.uniform(0, R + dist, per_loop)
+        ph = rng.uniform(0, 2 * np.pi, per_loop)
+        h = rng.uniform(-dist, dist, per_loop)
+        pts.append(c + r[:, None] * (np.cos(ph)[:, None] * e1 + np.sin(ph)[:, None] * e2)
+                   + h[:, None] * n)
+    return np.concatenate(pts)
+
+
+def test_c5_images_4(hip_ctx):
+    """200 loops drawn as the 4M-atom sample's, images +-4: 4,096 random atoms of
+    the box and 4,000 atoms within 6 A of the loop cuts (the near slab, where the
+    sum is exact in FP64) against the FP64 plain sum over all 729 images."""
+    s = G.config_c2(seed=20245, nloops=200)
+    rng = np.random.default_rng(4)
+    L = s.box[0]
+    P = np.concatenate([rng.uniform(-L / 2, L / 2, (4096, 3)),
+                        _near_cut_atoms(s.loops, rng, 20, 6.0)])
+    P = np.clip(P, -L / 2, L / 2 - 1e-9)          # inside the periodic box
+    x, y, z = _dev(P[:, 0], P[:, 1], P[:, 2])
+    ref = _plain64(hip_ctx, x, y, z, s.loops, s.box, 4)
+    cs = C.CubatureSum(hip_ctx, x, y, z, s.loops, s.box, 4, K_EXACT, near_tol=NEAR_TOL)
+    import torch
+    u = [torch.zeros_like(x) for _ in range(3)]
+    cs.run(x, y, z, *u)
+    torch.cuda.synchronize()
+    assert cs.eligible == 200 and cs.skipped > 0
+    cs.free()
+    e = _err(u, ref)
+    print(f"C5 cubature vs FP64 plain (+-4), {x.numel()} atoms: {e:.3e} A")
+    assert e < TOL
+
+
+def test_image_skip_leaves_exactly_those_images_out(hip_ctx):
+    """FP64: a plan with images skipped plus those images summed as translated
+    single-image loops equals the plain sum."""
+    import torch
+    rng = np.random.default_rng(9)
+    loops = kind_loops("glide", rng, 3) + kind_loops("prism", rng, 2)
+    x, y, z = _dev(*atoms_near(loops, rng, nrand=600, per_loop=60))
+    ref = _plain64(hip_ctx, x, y, z, loops, BOX, 1)
+    masks = np.zeros(len(loops), np.uint32)
+    extra = []
+    for l, lp in enumerate(loops):
+        for b in rng.choice(27, 9, replace=False):
+            i, j, k = b // 9 - 1, (b // 3) % 3 - 1, b % 3 - 1
+            masks[l] |= np.uint32(1 << int(b))
+            off = (i * BOX[0], j * BOX[1], k * BOX[2])
+            extra.append(B.LoopData(phys=lp.phys, line=True, vx=lp.vx + off[0],
+                                    vy=lp.vy + off[1], vz=lp.vz + off[2], cx=lp.cx + off[0],
+                                    cy=lp.cy + off[1], cz=lp.cz + off[2]))
+    p = B.Plan(hip_ctx, loops, BOX, 1, B.PREC_FP64)
+    hip_ctx.check(B.lib().disco_hip_plan_set_image_skip(
+        p._h, masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), len(loops)))
+    u = [torch.zeros_like(x) for _ in range(3)]
+    p.run_torch(x, y, z, *u)
+    q = B.Plan(hip_ctx, extra, BOX, 0, B.PREC_FP64)
+    q.run_torch(x, y, z, *u, accumulate=True)
+    torch.cuda.synchronize()
+    e = _err(u, ref)
+    assert e < 1e-10, e
+    # clearing the masks restores the plain sum
+    hip_ctx.check(B.lib().disco_hip_plan_set_image_skip(p._h, None, 0))
+    v = [torch.zeros_like(x) for _ in range(3)]
+    p.run_torch(x, y, z, *v)
+    torch.cuda.synchronize()
+    assert all(torch.equal(a, b) for a, b in zip(v, ref))
+    p.free()
+    q.free()
+
+
+@pytest.mark.parametrize("k_rule", [None, 8.0])
+def test_loop_forms_and_window(hip_ctx, k_rule):
+    """Glide, prismatic and general-b circles (the rule represents them), loops
+    off their plane (summed in global axes, exact) and rectangles (anisotropic:
+    exact), images +-3 in a small box; the hard switch and a blend 5 R .. 8 R."""
+    import torch
+    rng = np.random.default_rng(21)
+    loops = (kind_loops("glide", rng, 2) + kind_loops("prism", rng, 2) +
+             kind_loops("mixed", rng, 2) + kind_loops("bent", rng, 1))
+    V = np.array([[12, 7, 0], [-12, 7, 0], [-12, -7, 0], [12, -7, 0]], float) + [70, 80, 90]
+    loops.append(B.LoopData(phys=(2.5, 2.5, 2.5, 1, 0, 0, 0, 0, 1, 0.0), line=True,
+                            vx=V[:, 0].copy(), vy=V[:, 1].copy(), vz=V[:, 2].copy(), cx=70.0,
+                            cy=80.0, cz=90.0))
+    x, y, z = _dev(*atoms_near(loops, rng, nrand=3000, per_loop=100))
+    ref = _plain64(hip_ctx, x, y, z, loops, BOX, 3)
+    cs = C.CubatureSum(hip_ctx, x, y, z, loops, BOX, 3, K_EXACT, k_rule, near_tol=NEAR_TOL)
+    assert cs.eligible == 6                       # not the bent loop, not the rectangle
+    u = [torch.zeros_like(x) for _ in range(3)]
+    cs.run(x, y, z, *u)
+    torch.cuda.synchronize()
+    cs.free()
+    e = _err(u, ref)
+    print(f"loop forms, k_rule {k_rule}: {e:.3e} A")
+    assert e < TOL
+
+
+def test_cubature_off_is_the_plain_fp32c_pass_and_errors(hip_ctx):
+    import torch
+    s = G.config_c1()
+    idx = np.arange(0, s.n, 17)
+    x, y, z = _dev(s.x[idx], s.y[idx], s.z[idx])
+    p = B.Plan(hip_ctx, s.loops, s.box, 1, B.PREC_FP32C)
+    a = [torch.zeros_like(x) for _ in range(3)]
+    p.run_torch(x, y, z, *a)
+    assert p.set_cubature(5.0) == 2
+    assert p.set_cubature(0.0) == 0                # off again
+    b = [torch.zeros_like(x) for _ in range(3)]
+    p.run_torch(x, y, z, *b)
+    torch.cuda.synchronize()
+    assert all(torch.equal(s_, t) for s_, t in zip(a, b))
+    for bad in ((1.5, 2.0), (5.0, 4.0), (5.0, float("inf"))):
+        with pytest.raises(B.DiscoError):
+            p.set_cubature(*bad)
+    p.free()
+    p64 = B.Plan(hip_ctx, s.loops, s.box, 1, B.PREC_FP64)
+    with pytest.raises(B.DiscoError):
+        p64.set_cubature(5.0)
+    p64.free()
