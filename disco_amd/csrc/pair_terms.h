@@ -540,13 +540,13 @@ DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __rest
                              double bx, double by, double bz, double& ox, double& oy, double& oz)
 {
     const double l1sq = qx * qx + qy * qy + qz * qz;
-    const double rl1 = M::rsqrt_d(l1sq);
+    const double rl1 = M::rsqrt_d1(l1sq);
     const double l1 = l1sq * rl1;
 
     const SegD& last = s[nv - 1];
     double ax = last.bx - qx, ay = last.by - qy, az = last.bz - qz;
     const double la2 = ax * ax + ay * ay + az * az;
-    double rA = M::rsqrt_d(la2);
+    double rA = M::rsqrt_d1(la2);
     double RA = la2 * rA;
     double P = l1 * RA + (l1sq - (qx * last.bx + qy * last.by + qz * last.bz));
 
@@ -556,7 +556,7 @@ DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __rest
         const SegD& g = s[k];
         const double px = g.bx - qx, py = g.by - qy, pz = g.bz - qz;
         const double lb2 = px * px + py * py + pz * pz;
-        const double rB = M::rsqrt_d(lb2);
+        const double rB = M::rsqrt_d1(lb2);
         const double RB = lb2 * rB;
 
         const double r1b = l1sq - (qx * g.bx + qy * g.by + qz * g.bz);
@@ -586,7 +586,7 @@ DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __rest
             kd = dx * dx + dy * dy + dz * dz;
             kn = sB * rB - sA * rA;
         }
-        const double rr = M::rcp_d(RS * kd);
+        const double rr = M::rcp_d1(RS * kd);
         const double J0 = two_atanh_d<M>(g.L * kd * rr);
         const double K0 = kn * RS * rr;
         const double K1 = rA - rB;
@@ -623,11 +623,11 @@ template <class M>
 DISCO_HD double two_atanh_vote_d(double z)
 {
     const double y = z * z;
-    // z < 1/16: the series to z^15 (the next term is below 2^-60 of the sum);
-    // the one to z^27 for z < 1/4 only when a lane of the wave needs it, so
-    // each lane's result depends on its own z alone
-    double p = 1.0 / 15;
-    p = p * y + 1.0 / 13; p = p * y + 1.0 / 11; p = p * y + 1.0 / 9;
+    // z < 1/16: the series to z^13 (the first omitted term, 2 z^15 / 15, is
+    // below 2^-59 of the sum); the one to z^27 for z < 1/4 only when a lane of
+    // the wave needs it, so each lane's result depends on its own z alone
+    double p = 1.0 / 13;
+    p = p * y + 1.0 / 11; p = p * y + 1.0 / 9;
     p = p * y + 1.0 / 7;  p = p * y + 1.0 / 5;  p = p * y + 1.0 / 3;
     const bool mid = !(z < 0.0625);
     if (__builtin_expect(M::any(mid), 0)) {
@@ -647,13 +647,15 @@ DISCO_HD double two_atanh_vote_d(double z)
     return r;
 }
 
-template <class M>
+// GLIDE: b in the loop plane, so b x t is along n (w1 = w2 = 0) and a.w =
+// -h w3: the factor -h is applied once after the walk, as in the FP32 form.
+template <class M, bool GLIDE>
 DISCO_HD void flat_step_d(const SegPD& g, double q1, double q2, double h2, double u1, double u2,
                           double uh2, double nh, double mh, FlatWalkD& w)
 {
     const double p1 = g.b1 - q1, p2 = g.b2 - q2;
     const double lb2 = p1 * p1 + (p2 * p2 + h2);
-    const double rB = M::rsqrt_d(lb2);
+    const double rB = M::rsqrt_d1(lb2);
     const double RB = lb2 * rB;
     const double rArB = w.rA * rB;
 
@@ -680,15 +682,17 @@ DISCO_HD void flat_step_d(const SegPD& g, double q1, double q2, double h2, doubl
         kd = mixed ? dd : kd;
         kn = mixed ? knm : kn;
     }
-    const double rr = M::rcp_d(RS * kd);
+    const double rr = M::rcp_d1(RS * kd);
     const double J0 = two_atanh_vote_d<M>(g.L * kd * rr);
     const double K0 = kn * RS * rr;
     const double K1 = w.rA - rB;
-    const double dw = w.a1 * g.w1 + (w.a2 * g.w2 + mh * g.w3);
+    const double dw = GLIDE ? g.w3 : w.a1 * g.w1 + (w.a2 * g.w2 + mh * g.w3);
     const double al = dw * K0;
     const double be = dw * (K1 - sA * K0);
-    w.sw1 += J0 * g.w1;
-    w.sw2 += J0 * g.w2;
+    if (!GLIDE) {
+        w.sw1 += J0 * g.w1;
+        w.sw2 += J0 * g.w2;
+    }
     w.sw3 += J0 * g.w3;
     w.x1 = (w.x1 + al * w.a1) + be * g.t1;
     w.x2 = (w.x2 + al * w.a2) + be * g.t2;
@@ -700,12 +704,12 @@ DISCO_HD void flat_step_d(const SegPD& g, double q1, double q2, double h2, doubl
 
 // Field of one image in the loop frame: fo = Omega / 4 pi (the caller adds
 // b fo) and o = cJ sum J0 w - cK sum (al a + be t).
-template <class M>
+template <class M, bool GLIDE = false>
 DISCO_HD void line_image_flat_f64(double q1, double q2, double h, const SegPD* __restrict__ s,
                                   int nv, double& fo, double& o1, double& o2, double& o3)
 {
     const double h2 = h * h;
-    const double rl1 = M::rsqrt_d(q1 * q1 + (q2 * q2 + h2));
+    const double rl1 = M::rsqrt_d1(q1 * q1 + (q2 * q2 + h2));
     const double u1 = q1 * rl1, u2 = q2 * rl1, uh2 = h2 * rl1, nh = -h * rl1, mh = -h;
 
     FlatWalkD w;
@@ -713,7 +717,7 @@ DISCO_HD void line_image_flat_f64(double q1, double q2, double h, const SegPD* _
     w.a1 = last.b1 - q1;
     w.a2 = last.b2 - q2;
     const double la2 = w.a1 * w.a1 + (w.a2 * w.a2 + h2);
-    w.rA = M::rsqrt_d(la2);
+    w.rA = M::rsqrt_d1(la2);
     w.RA = la2 * w.rA;
     w.P = w.RA + (uh2 - (u1 * w.a1 + u2 * w.a2));
     w.X = 1.0; w.Y = 0.0;
@@ -721,16 +725,24 @@ DISCO_HD void line_image_flat_f64(double q1, double q2, double h, const SegPD* _
     w.x1 = w.x2 = w.sal = 0.0;
     for (int k0 = 0; k0 < nv; k0 += 64) {
         const int k1 = nv < k0 + 64 ? nv : k0 + 64;
+        // two segments per iteration: no register copies of the carried A - p
+        int k = k0;
 #pragma unroll 1
-        for (int k = k0; k < k1; k++) flat_step_d<M>(s[k], q1, q2, h2, u1, u2, uh2, nh, mh, w);
+        for (; k + 1 < k1; k += 2) {
+            flat_step_d<M, GLIDE>(s[k], q1, q2, h2, u1, u2, uh2, nh, mh, w);
+            flat_step_d<M, GLIDE>(s[k + 1], q1, q2, h2, u1, u2, uh2, nh, mh, w);
+        }
+        if (k < k1) flat_step_d<M, GLIDE>(s[k], q1, q2, h2, u1, u2, uh2, nh, mh, w);
         const int e = M::exponent_d(M::fmax_d(M::fabs_d(w.X), M::fabs_d(w.Y)));
         w.X = M::ldexp_d(w.X, -e);
         w.Y = M::ldexp_d(w.Y, -e);
     }
     fo = 2.0 * atan2_own_d<M>(w.Y, w.X) * kInv4Pi;
-    o1 = kCJ * w.sw1 - kC3 * w.x1;
-    o2 = kCJ * w.sw2 - kC3 * w.x2;
-    o3 = kCJ * w.sw3 - kC3 * (mh * w.sal);
+    double x1 = w.x1, x2 = w.x2, x3 = mh * w.sal;
+    if (GLIDE) { x1 *= mh; x2 *= mh; x3 *= mh; }
+    o1 = kCJ * w.sw1 - kC3 * x1;
+    o2 = kCJ * w.sw2 - kC3 * x2;
+    o3 = kCJ * w.sw3 - kC3 * x3;
 }
 
 // ------------------------------------------------------------ loopons
@@ -813,7 +825,7 @@ DISCO_HD void loopon_image_f64(double qx, double qy, double qz, const double4* _
         const double rx = qx - v.x, ry = qy - v.y, rz = qz - v.z;
         const double r2 = rx * rx + ry * ry + rz * rz;
         const bool keep = !(r2 < 1e-20);                // r < 1e-10, Code/kroupa.hpp:59
-        const double ri = M::rsqrt_d(keep ? r2 : 1.0);
+        const double ri = M::rsqrt_d1(keep ? r2 : 1.0);
         const double ri2 = ri * ri;
         const double ri3 = ri2 * ri;
         const double mr = mx * rx + my * ry + mz * rz;
