@@ -39,8 +39,8 @@ FP32_UNPACKED_PEAK_TFLOPS = 78.6  # the same without packed FP32 (one FMA per la
 FAR_PASS_HBM_BYTES_PER_ATOM = 48.49
 # cubature form of the sample (reported beside the plain sum): exact within 5 cut
 # radii of a loop image, the 9-point cubature beyond; +-1 images far from every
-# atom on an L/64 grid (interpolation error bound 1e-7 A each), the far shell on
-# an L/24 grid
+# atom on an L/32 grid (interpolation error bound 1e-7 A each), the far shell on
+# an L/16 grid, both interpolated by 6-node stencils
 K_EXACT, NEAR_TOL = 5.0, 1e-7
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on this workload: "
                   "profiles/line_kernel_far__fcc4m_50loops_img2__pmc_traffic.md")
@@ -219,7 +219,7 @@ def main():
               f"{ms64:.1f} ms ({fp64['evals_per_s']:.3e} evals/s)", flush=True)
 
     # the same sample in cubature form (exact within K_EXACT loop radii of a loop
-    # image, the 7-point cubature beyond, far images on grids): ms per sample and
+    # image, the 9-point cubature beyond, far images on grids): ms per sample and
     # its error against the FP64 plain sum over every atom
     cub = None
     if not args.no_cubature:
@@ -241,7 +241,7 @@ def main():
                "rule_loops": cs.eligible, "images_on_grid": cs.skipped,
                "form": "exact line integral within k_exact R of each loop image (FP32, FP64 "
                        "near slab), 9-point cubature of the cut beyond; +-1 images far from "
-                       "every atom on an L/64 grid, the far shell on an L/24 grid"}
+                       "every atom on an L/32 grid, the far shell on an L/16 grid (6-node stencils)"}
         cs.free()
         del w
         print(f"[bench] cubature form: {cms:.1f} ms per sample ({ms_step / cms:.1f}x the plain "

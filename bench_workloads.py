@@ -188,7 +188,8 @@ def run_c5(args):
     """The 200-loop sample at images +-4 (729 per loop) in cubature form: exact
     line integral within 5 cut radii of each loop image (FP32, FP64 near slab),
     the 9-point cubature of the cut beyond, the +-1 images far from every atom
-    on an L/64 grid and the far shell (images 2..4) on an L/24 grid.
+    on an L/32 grid and the far shell (images 2..4) on an L/16 grid (6-node
+    interpolation stencils).
     Error against the FP64 plain sum on every 8th atom (every atom with
     --c5-all); the plain GPU sum timed on a contiguous eighth of the atoms and
     scaled; the oracle timed on a few atoms."""

@@ -178,7 +178,7 @@ def lib():
         L.disco_hip_lattice_atoms.restype = ctypes.c_longlong
         L.disco_hip_lattice.argtypes = [_vp, _i, _i, _i, _i, _d, _vp, _vp, _vp, _vp]
         L.disco_hip_grid_interp.argtypes = [_vp, _i] + [_vp] * 9 + [_i, _i, _i, _dp, _d, _i,
-                                                                     _vp]
+                                                                     _i, _vp]
         _lib = L
     return _lib
 
@@ -221,10 +221,10 @@ def lattice_torch(ctx, kind: str, nx: int, ny: int, nz: int, a0: float, out=None
 
 
 # ------------------------------------------------------------------ grid field
-def grid_interp_torch(ctx, x, y, z, ux, uy, uz, g, origin, h, accumulate=True):
+def grid_interp_torch(ctx, x, y, z, ux, uy, uz, g, origin, h, accumulate=True, order=4):
     """u (+)= the field g (three device tensors of shape (nz, ny, nx), x fastest;
     node (i, j, k) at origin + (i, j, k) h) interpolated at the device atoms by
-    tensor-product cubic Lagrange."""
+    tensor-product Lagrange on `order` nodes per axis (4: cubic, 6: quintic)."""
     import torch
     gz_, gy_, gx_ = g[0].shape
     gc = [t.contiguous() for t in g]
@@ -233,7 +233,7 @@ def grid_interp_torch(ctx, x, y, z, ux, uy, uz, g, origin, h, accumulate=True):
     ctx.check(lib().disco_hip_grid_interp(
         ctx._h, x.numel(), x.data_ptr(), y.data_ptr(), z.data_ptr(), ux.data_ptr(),
         uy.data_ptr(), uz.data_ptr(), gc[0].data_ptr(), gc[1].data_ptr(), gc[2].data_ptr(),
-        gx_, gy_, gz_, o, float(h), 1 if accumulate else 0, stream))
+        gx_, gy_, gz_, o, float(h), int(order), 1 if accumulate else 0, stream))
 
 
 # ------------------------------------------------------------------ devices

@@ -556,8 +556,28 @@ __global__ __launch_bounds__(kBlock) void lattice_kernel(int ny, int nz, double 
 }
 
 // ------------------------------------------------------------------ grid field
-// 4-node cubic Lagrange weights at fractional position f in [0, 1) of the
-// interval between nodes 1 and 2 of the stencil {0, 1, 2, 3}.
+// Lagrange weights of the ORDER-node stencil {0, .., ORDER-1} at fractional
+// position f in [0, 1) of its central interval (between nodes ORDER/2 - 1
+// and ORDER/2): w_j = prod_{k != j} (f + ORDER/2 - 1 - k) / (j - k).
+template <int ORDER>
+__device__ __forceinline__ void lagrange_weights(double f, double w[ORDER])
+{
+    const double t = f + (ORDER / 2 - 1);          // position in node units
+    double d[ORDER];
+#pragma unroll
+    for (int k = 0; k < ORDER; k++) d[k] = t - k;
+#pragma unroll
+    for (int j = 0; j < ORDER; j++) {
+        double num = 1.0, den = 1.0;
+#pragma unroll
+        for (int k = 0; k < ORDER; k++)
+            if (k != j) { num *= d[k]; den *= (double)(j - k); }
+        w[j] = num / den;
+    }
+}
+
+// 4-node cubic weights in the closed form of the cubic interpolant (kept as
+// it is: the far-shell results of the 4-node grid stay bit for bit)
 __device__ __forceinline__ void cubic_weights(double f, double w[4])
 {
     const double fm1 = f - 1.0, fm2 = f - 2.0, fp1 = f + 1.0;
@@ -567,14 +587,18 @@ __device__ __forceinline__ void cubic_weights(double f, double w[4])
     w[3] = fp1 * f * fm1 * (1.0 / 6.0);
 }
 
+// First node of the ORDER-node stencil of position t (node units) on an axis of
+// nn nodes, kept inside the grid; f = t - (its central interval's start).
+template <int ORDER>
 __device__ __forceinline__ int stencil_start(double t, int nn, double& f)
 {
     double c = floor(t);
-    c = fmin(fmax(c, 1.0), (double)(nn - 3));     // nodes c-1 .. c+2 inside [0, nn)
+    c = fmin(fmax(c, (double)(ORDER / 2 - 1)), (double)(nn - ORDER / 2 - 1));
     f = t - c;
-    return (int)c - 1;
+    return (int)c - (ORDER / 2 - 1);
 }
 
+template <int ORDER>
 __global__ __launch_bounds__(kBlock) void grid_interp_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
@@ -583,21 +607,27 @@ __global__ __launch_bounds__(kBlock) void grid_interp_kernel(
 {
     const int a = blockIdx.x * kBlock + threadIdx.x;
     if (a >= n) return;
-    double fx, fy, fz, wx[4], wy[4], wz[4];
-    const int i0 = stencil_start((x[a] - ox) * ih, nx, fx);
-    const int j0 = stencil_start((y[a] - oy) * ih, ny, fy);
-    const int k0 = stencil_start((z[a] - oz) * ih, nz, fz);
-    cubic_weights(fx, wx);
-    cubic_weights(fy, wy);
-    cubic_weights(fz, wz);
+    double fx, fy, fz, wx[ORDER], wy[ORDER], wz[ORDER];
+    const int i0 = stencil_start<ORDER>((x[a] - ox) * ih, nx, fx);
+    const int j0 = stencil_start<ORDER>((y[a] - oy) * ih, ny, fy);
+    const int k0 = stencil_start<ORDER>((z[a] - oz) * ih, nz, fz);
+    if (ORDER == 4) {
+        cubic_weights(fx, wx);
+        cubic_weights(fy, wy);
+        cubic_weights(fz, wz);
+    } else {
+        lagrange_weights<ORDER>(fx, wx);
+        lagrange_weights<ORDER>(fy, wy);
+        lagrange_weights<ORDER>(fz, wz);
+    }
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int k = 0; k < 4; k++)
-        for (int j = 0; j < 4; j++) {
+    for (int k = 0; k < ORDER; k++)
+        for (int j = 0; j < ORDER; j++) {
             const double wjk = wy[j] * wz[k];
             const long long row = ((long long)(k0 + k) * ny + (j0 + j)) * nx + i0;
             double rx = 0.0, ry = 0.0, rz = 0.0;
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
+            for (int i = 0; i < ORDER; i++) {
                 rx += wx[i] * gx[row + i];
                 ry += wx[i] * gy[row + i];
                 rz += wx[i] * gz[row + i];
@@ -1207,22 +1237,24 @@ int disco_hip_line_fp64(void) { return 1; }
 int disco_hip_grid_interp(disco_ctx* ctx, int n, const double* d_x, const double* d_y,
                           const double* d_z, double* d_ux, double* d_uy, double* d_uz,
                           const double* d_gx, const double* d_gy, const double* d_gz, int nx,
-                          int ny, int nz, const double* origin, double h, int accumulate,
-                          void* stream)
+                          int ny, int nz, const double* origin, double h, int order,
+                          int accumulate, void* stream)
 {
     if (!ctx) return DISCO_ERR_ARG;
     if (n < 0) return fail(ctx, DISCO_ERR_ARG, "negative size");
-    if (nx < 4 || ny < 4 || nz < 4) return fail(ctx, DISCO_ERR_ARG, "a grid needs >= 4 nodes per axis");
+    if (order != 4 && order != 6) return fail(ctx, DISCO_ERR_ARG, "stencil order must be 4 or 6");
+    if (nx < order || ny < order || nz < order)
+        return fail(ctx, DISCO_ERR_ARG, "a grid needs at least `order` nodes per axis");
     if (!origin || !(h > 0.0) || !std::isfinite(h))
         return fail(ctx, DISCO_ERR_ARG, "bad grid origin or spacing");
     if (n == 0) return DISCO_OK;
     if (!d_x || !d_y || !d_z || !d_ux || !d_uy || !d_uz || !d_gx || !d_gy || !d_gz)
         return fail(ctx, DISCO_ERR_ARG, "null device array");
     DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
-    hipLaunchKernelGGL(grid_interp_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)),
-                       dim3(kBlock), 0, (hipStream_t)stream, d_x, d_y, d_z, d_ux, d_uy, d_uz, n,
-                       d_gx, d_gy, d_gz, nx, ny, nz, origin[0], origin[1], origin[2], 1.0 / h,
-                       accumulate);
+    auto kern = order == 6 ? grid_interp_kernel<6> : grid_interp_kernel<4>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       (hipStream_t)stream, d_x, d_y, d_z, d_ux, d_uy, d_uz, n, d_gx, d_gy, d_gz,
+                       nx, ny, nz, origin[0], origin[1], origin[2], 1.0 / h, accumulate);
     DISCO_HIP_TRY(ctx, hipGetLastError());
     return DISCO_OK;
 }

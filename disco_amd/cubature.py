@@ -18,7 +18,7 @@ hexagons).  The cubature form is
 * the far shell (images with max(|i|, |j|, |k|) in (1, img]), at least one box
   length from every atom, by the same cubature summed by the loopon kernels at
   the nodes of a grid of spacing ``grid_h`` over the atoms and interpolated to
-  them by cubic Lagrange (``disco_hip_grid_interp``); so are the +-1 images so
+  them by 6-node Lagrange (``disco_hip_grid_interp``); so are the +-1 images so
   far from every atom that the cubature alone is accurate at all of them and
   a finer grid interpolates their field to a stated tolerance (``far_images``).
 
@@ -37,20 +37,23 @@ from . import surrogate as S
 
 
 # Interpolation error model of a loop image's field on a grid of spacing h by
-# tensor-product cubic Lagrange at distance r from the image's centre: the
-# field is about 0.31 |b| (R/r)^2 there, its fourth derivatives about 120 times
-# that over r^4, and the 4-node interpolant errs by at most 9/384 h^4 f^(4)
-# per axis: error <= GRID_ERR |b| R^2 h^4 / r^6 (an upper bound by a factor of
-# about 2.5 on the 4M-atom samples, measured on the GPU).
-GRID_ERR = 2.6
+# tensor-product Lagrange on p nodes per axis, at distance r from the image's
+# centre: the field is about 0.31 |b| (R/r)^2 there, its p-th derivatives
+# about (p + 1)! times that over r^p, and the p-node interpolant errs by at
+# most max|prod (t - t_i)| / p! h^p f^(p) per axis (9/16 and 225/64 for p = 4
+# and 6): error <= GRID_ERR[p] |b| R^2 h^p / r^(p+2) over three axes (for p = 4
+# an upper bound by a factor of about 2.5 on the 4M-atom samples, measured on
+# the GPU).
+GRID_ERR = {4: 3 * 0.31 * 120 * (9 / 16) / 24, 6: 3 * 0.31 * 5040 * (225 / 64) / 720}
 
 
-def far_images(loops, box, lo, hi, h, tol, k_exact, dmin=0.0, info=None):
+def far_images(loops, box, lo, hi, h, tol, k_exact, dmin=0.0, info=None, order=6):
     """Images of the +-1 shell to sum on a grid of spacing h instead of at the
     atoms (loops whose cut the cubature represents only): those whose centre
-    lies at least max(k_exact R, R + dmin, (GRID_ERR |b| R^2 h^4 / tol)^(1/6))
-    from the box [lo, hi] holding the atoms, so that the cubature alone is
-    accurate at every atom and the grid's interpolation error is below tol.
+    lies at least max(k_exact R, R + dmin, (GRID_ERR |b| R^2 h^p / tol)^(1/(p+2)))
+    (p = order) from the box [lo, hi] holding the atoms, so that the cubature
+    alone is accurate at every atom and the grid's interpolation error is below
+    tol.
     Returns the skip mask of each loop (bit 9 (i + 1) + 3 (j + 1) + (k + 1))
     and the list of (loop index, offset) of those images, offset =
     (i Lx, j Ly, k Lz) (the image of a loop centred at c sits at c + offset)."""
@@ -64,7 +67,8 @@ def far_images(loops, box, lo, hi, h, tol, k_exact, dmin=0.0, info=None):
         c = np.array([lp.cx, lp.cy, lp.cz])
         R = float(info[l, 4])
         bm = float(np.linalg.norm(lp.burgers()))
-        dmin_l = max(k_exact * R, R + dmin, (GRID_ERR * bm * R * R * h ** 4 / tol) ** (1 / 6))
+        dmin_l = max(k_exact * R, R + dmin,
+                     (GRID_ERR[order] * bm * R * R * h ** order / tol) ** (1 / (order + 2)))
         for i, j, k in itertools.product((-1, 0, 1), repeat=3):
             if i == j == k == 0:
                 continue
@@ -79,12 +83,12 @@ def far_images(loops, box, lo, hi, h, tol, k_exact, dmin=0.0, info=None):
 
 class _Grid:
     """Loopon plans summed at the nodes of a uniform grid over the atoms and
-    interpolated to them (tensor-product cubic Lagrange)."""
+    interpolated to them (tensor-product Lagrange on `order` nodes per axis)."""
 
-    def __init__(self, ctx, x, y, z, box, h, parts, groups=None):
+    def __init__(self, ctx, x, y, z, box, h, parts, groups=None, order=6):
         import torch
-        self.ctx, self.h = ctx, float(h)
-        self.origin, dims, self.nodes = S.grid_nodes(x, y, z, self.h)
+        self.ctx, self.h, self.order = ctx, float(h), int(order)
+        self.origin, dims, self.nodes = S.grid_nodes(x, y, z, self.h, margin=self.order // 2 - 1)
         tiles = (self.nodes[0].numel() + 255) // 256
         self.plans = []
         for lps, img, inner in parts:
@@ -101,7 +105,7 @@ class _Grid:
         for k, p in enumerate(self.plans):
             p.run_torch(*self.nodes, *self.gu, accumulate=k > 0)
         B.grid_interp_torch(self.ctx, x, y, z, ux, uy, uz, [t.view(self.shape) for t in self.gu],
-                            self.origin, self.h, accumulate=True)
+                            self.origin, self.h, accumulate=True, order=self.order)
 
     def free(self):
         for p in self.plans:
@@ -116,15 +120,16 @@ class CubatureSum:
 
     near_tol (A): images of the +-1 shell far enough from every atom that the
     cubature alone is accurate there and a grid of spacing near_h (default
-    L/64) interpolates their field to within near_tol (far_images) are taken
+    L/32) interpolates their field to within near_tol (far_images) are taken
     out of the line plan and summed as cubatures at that grid's nodes (None:
     every +-1 image stays in the line plan).  The far shell (1, img] is summed
-    as cubatures on a grid of spacing grid_h (default L/24).  Defaults: measured
-    on the 4M-atom samples at images +-2 and +-4 (max |du| 1.1e-6 and 2.4e-6 A
-    against the FP64 plain sum)."""
+    as cubatures on a grid of spacing grid_h (default L/16).  Both grids are
+    interpolated by 6-node (quintic) stencils (order = 4: cubic, with defaults
+    L/64 and L/24).  Defaults: measured on the 4M-atom samples at images +-2
+    and +-4 (max |du| 1.0e-6 and 2.0e-6 A against the FP64 plain sum)."""
 
     def __init__(self, ctx, x, y, z, loops, box, img, k_exact=5.0, k_rule=None,
-                 grid_h=None, grid_groups=None, near_tol=1e-7, near_h=None):
+                 grid_h=None, grid_groups=None, near_tol=1e-7, near_h=None, order=6):
         self.ctx = ctx
         self.img = int(img)
         self.near = B.Plan(ctx, loops, box, min(1, self.img), B.PREC_FP32C)
@@ -132,15 +137,15 @@ class CubatureSum:
         self.nloops = len(loops)
         self.grids = []
         self.skipped = 0
-        hf = float(grid_h) if grid_h else min(box) / 24.0
+        hf = float(grid_h) if grid_h else min(box) / (16.0 if order == 6 else 24.0)
         parts = {}                 # grid spacing -> [(loopon loops, img, inner)]
         base = None
         if near_tol is not None and self.img >= 1:
-            hn = float(near_h) if near_h else min(box) / 64.0
+            hn = float(near_h) if near_h else min(box) / (32.0 if order == 6 else 64.0)
             lo = [float(t.min()) for t in (x, y, z)]
             hi = [float(t.max()) for t in (x, y, z)]
             masks, far = far_images(loops, box, lo, hi, hn, float(near_tol),
-                                    max(float(k_exact), float(k_rule or 0.0)))
+                                    max(float(k_exact), float(k_rule or 0.0)), order=order)
             if far:
                 ctx.check(B.lib().disco_hip_plan_set_image_skip(
                     self.near._h, masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
@@ -157,7 +162,7 @@ class CubatureSum:
             base = S.base_loopons(loops, "rule9") if base is None else base
             parts.setdefault(hf, []).append((base, self.img, 1))
         for h, pl in parts.items():
-            self.grids.append(_Grid(ctx, x, y, z, box, h, pl, grid_groups))
+            self.grids.append(_Grid(ctx, x, y, z, box, h, pl, grid_groups, order))
 
     def run(self, x, y, z, ux, uy, uz, accumulate=False):
         self.near.run_torch(x, y, z, ux, uy, uz, accumulate=accumulate)

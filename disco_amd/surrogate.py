@@ -355,16 +355,17 @@ def exact_far_shell_torch(ctx, x, y, z, loops, box, img, prec=B.PREC_FP32C):
     return u
 
 
-def grid_nodes(x, y, z, h):
-    """Uniform grid of spacing h covering the device atoms with one node of margin
-    on each side (the 4-node stencils of the interpolation stay inside): returns
-    the origin, the node counts (nx, ny, nz) and the node coordinates (three
-    device tensors of nz * ny * nx, x fastest)."""
+def grid_nodes(x, y, z, h, margin=1):
+    """Uniform grid of spacing h covering the device atoms with `margin` nodes of
+    margin on each side (1 keeps the 4-node stencils of the interpolation
+    inside, 2 the 6-node ones): returns the origin, the node counts (nx, ny,
+    nz) and the node coordinates (three device tensors of nz * ny * nx, x
+    fastest)."""
     import torch
     lo = [float(t.min()) for t in (x, y, z)]
     hi = [float(t.max()) for t in (x, y, z)]
-    dims = [int(math.ceil((b - a) / h)) + 4 for a, b in zip(lo, hi)]
-    origin = [a - h for a in lo]
+    dims = [int(math.ceil((b - a) / h)) + 2 * margin + 2 for a, b in zip(lo, hi)]
+    origin = [a - margin * h for a in lo]
     axes = [origin[k] + h * torch.arange(dims[k], dtype=torch.float64, device=x.device)
             for k in range(3)]
     gz, gy, gx = torch.meshgrid(axes[2], axes[1], axes[0], indexing="ij")

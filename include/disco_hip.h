@@ -318,16 +318,17 @@ int disco_hip_lattice(disco_ctx *ctx, int lattice, int nx, int ny, int nz, doubl
 /* ---------------------------------------------------------------- grid field
  * (No counterpart in the reference.)  u += (accumulate = 1) or u = (0) a field
  * tabulated on a uniform grid, interpolated at n device atoms by the tensor
- * product of 4-node cubic Lagrange interpolants (exact for polynomials of
- * degree <= 3 along each axis).  g_x, g_y, g_z: nx * ny * nz doubles each, x
- * fastest; node (i, j, k) sits at origin + (i, j, k) h.  nx, ny, nz >= 4; an
- * atom outside the nodes' span uses the nearest 4-node stencil.  Used for the
- * smooth far image shell of a wide image sum.  Asynchronous on `stream`. */
+ * product of `order`-node Lagrange interpolants: order 4 (cubic, exact for
+ * polynomials of degree <= 3 along each axis) or 6 (quintic, degree <= 5).
+ * g_x, g_y, g_z: nx * ny * nz doubles each, x fastest; node (i, j, k) sits at
+ * origin + (i, j, k) h.  nx, ny, nz >= order; an atom outside the nodes' span
+ * uses the nearest stencil.  Used for the smooth far images of a wide image
+ * sum.  Asynchronous on `stream`. */
 int disco_hip_grid_interp(disco_ctx *ctx, int n, const double *d_x, const double *d_y,
                           const double *d_z, double *d_ux, double *d_uy, double *d_uz,
                           const double *d_gx, const double *d_gy, const double *d_gz, int nx,
-                          int ny, int nz, const double *origin, double h, int accumulate,
-                          void *stream);
+                          int ny, int nz, const double *origin, double h, int order,
+                          int accumulate, void *stream);
 
 #ifdef __cplusplus
 }

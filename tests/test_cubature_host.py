@@ -69,16 +69,16 @@ def test_rule_matches_the_numpy_construction():
 
 def test_far_image_selection():
     """An image goes to the grid when its centre lies at least max(k R, R + dmin,
-    (GRID_ERR |b| R^2 h^4 / tol)^(1/6)) from the atom box; bit 9 (i + 1) +
-    3 (j + 1) + (k + 1)."""
+    (GRID_ERR |b| R^2 h^p / tol)^(1/(p+2))) from the atom box (p-node stencils);
+    bit 9 (i + 1) + 3 (j + 1) + (k + 1)."""
     lp = _polygon(48, 10.0, c=(40.0, 0.0, 0.0))
     box = (100.0, 100.0, 100.0)
     lo, hi = (-50.0, -50.0, -50.0), (50.0, 50.0, 50.0)
     h, tol, k = 2.0, 1e-7, 6.0
-    masks, far = C.far_images([lp], box, lo, hi, h, tol, k, dmin=30.0)
+    masks, far = C.far_images([lp], box, lo, hi, h, tol, k, dmin=30.0, order=4)
     got = {tuple(int(round(t / 100.0)) for t in off) for _, off in far}
     R = C.B.line_cubature([lp])[0, 4]
-    dmin = max(k * R, R + 30.0, (C.GRID_ERR * 2.5 * R * R * h ** 4 / tol) ** (1 / 6))
+    dmin = max(k * R, R + 30.0, (C.GRID_ERR[4] * 2.5 * R * R * h ** 4 / tol) ** (1 / 6))
     want = set()
     for i in (-1, 0, 1):
         for j in (-1, 0, 1):
@@ -93,8 +93,12 @@ def test_far_image_selection():
     bits = {b for b in range(27) if (int(masks[0]) >> b) & 1}
     assert bits == {9 * (i + 1) + 3 * (j + 1) + (kk + 1) for i, j, kk in want}
     # a coarser grid (larger interpolation error) takes fewer images
-    m_coarse, f_coarse = C.far_images([lp], box, lo, hi, 20.0, tol, k, dmin=30.0)
+    m_coarse, f_coarse = C.far_images([lp], box, lo, hi, 20.0, tol, k, dmin=30.0, order=4)
     assert len(f_coarse) < len(far)
+    # on a grid of spacing 5 a 6-node stencil errs less than the 4-node one: more images
+    m4, f4 = C.far_images([lp], box, lo, hi, 5.0, tol, k, dmin=30.0, order=4)
+    m6, f6 = C.far_images([lp], box, lo, hi, 5.0, tol, k, dmin=30.0, order=6)
+    assert len(f6) > len(f4)
     # loops the rule does not represent are never taken out
     m2, f2 = C.far_images([_rect(5, 3)], box, lo, hi, h, tol, k)
     assert int(m2[0]) == 0 and not f2

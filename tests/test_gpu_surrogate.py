@@ -181,6 +181,36 @@ def test_grid_interp_reproduces_cubics(hip_ctx):
     assert np.abs(u[0].cpu().numpy() - 2 * want).max() <= 1e-10 * np.abs(want).max()
 
 
+def test_grid_interp_quintic_reproduces_degree_5(hip_ctx):
+    """The 6-node stencil is exact for degree <= 5 per axis; its grid needs 6 nodes
+    per axis, and orders other than 4 and 6 are refused."""
+    import torch
+    rng = np.random.default_rng(9)
+    h, origin, dims = 1.3, (1.0, -2.0, 0.25), (11, 9, 13)
+    f = lambda X, Y, Z: ((1 - X + 0.2 * X ** 3 - 0.01 * X ** 5) * (2 + Y ** 4 * 0.01 - Y)
+                         * (0.5 + 0.001 * Z ** 5 - 0.02 * Z ** 2))
+    ax = [origin[k] + h * np.arange(dims[k]) for k in range(3)]
+    Zg, Yg, Xg = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    g = [torch.from_numpy(np.ascontiguousarray(c * f(Xg, Yg, Zg))).cuda() for c in (1.0, 3.0, -1.0)]
+    P = np.stack([rng.uniform(origin[k] + 2 * h, origin[k] + h * (dims[k] - 3), 700)
+                  for k in range(3)], 1)
+    x, y, z = (torch.from_numpy(np.ascontiguousarray(P[:, k])).cuda() for k in range(3))
+    u = [torch.zeros_like(x) for _ in range(3)]
+    B.grid_interp_torch(hip_ctx, x, y, z, *u, g, origin, h, accumulate=False, order=6)
+    want = f(P[:, 0], P[:, 1], P[:, 2])
+    for c, t in zip((1.0, 3.0, -1.0), u):
+        assert np.abs(t.cpu().numpy() - c * want).max() <= 1e-10 * np.abs(want).max()
+    # the cubic stencil is not exact for this field
+    v = [torch.zeros_like(x) for _ in range(3)]
+    B.grid_interp_torch(hip_ctx, x, y, z, *v, g, origin, h, accumulate=False, order=4)
+    assert np.abs(v[0].cpu().numpy() - want).max() > 1e-6 * np.abs(want).max()
+    small = [t[:5, :5, :5].contiguous() for t in g]
+    with pytest.raises(B.DiscoError):
+        B.grid_interp_torch(hip_ctx, x, y, z, *u, small, origin, h, order=6)
+    with pytest.raises(B.DiscoError):
+        B.grid_interp_torch(hip_ctx, x, y, z, *u, g, origin, h, order=5)
+
+
 @pytest.mark.parametrize("div,tol64", [(32, 2e-7), (24, 6e-7)])
 def test_grid_far_shell_matches_the_direct_sum(hip_ctx, div, tol64):
     """The rule's far shell (images 2..4) tabulated on an L/32 (L/24) grid over
