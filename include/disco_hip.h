@@ -244,13 +244,18 @@ int disco_hip_batch_free(disco_batch *batch);
 
 /* ---------------------------------------------------------------- surrogate
  * The learned far image shell.  The field of a loop over the images with
- * max(|i|,|j|,|k|) in (1, img] is the far shell; it is the far-shell sum of
- * the loop's dipole term (one loopon of the loop's area at its centre, summed
- * exactly by a loopon plan with disco_hip_plan_set_inner) plus a residual that
- * an MLP predicts: 14 features -> 128 -> 128 -> 128 -> 3 (SiLU), layer 1 in
- * FP32, layers 2-4 BF16 on MFMA with FP32 accumulation, output scaled per loop
- * by disco_hip_surrogate_scale.  It is trained (Adam, FP32 masters) by the
- * library's own forward and backward kernels.  No counterpart in the
+ * max(|i|,|j|,|k|) in (1, img] is the far shell.  It is modelled as a baseline
+ * rule, the loop's cut as a few weighted loopons summed over the far shell by a
+ * loopon plan with disco_hip_plan_set_inner (one loopon of the loop's area, or
+ * the centre and a ring of six matching the cut's area and radial moments, on
+ * a grid with disco_hip_grid_interp), plus a residual that an MLP predicts:
+ * 14 features -> 128 -> 128 -> 128 -> 3 (hard swish), layer 1 in FP32, layers
+ * 2-4 BF16 on MFMA with FP32 accumulation, output scaled per loop by
+ * disco_hip_surrogate_scale.  It is trained (Adam, FP32 masters) by the
+ * library's own forward and backward kernels.  The host pipeline accepts the
+ * surrogate only when its error on a validation sample of atoms against the
+ * exact far shell is within the tolerance, else sums the exact far shell; the
+ * validation is statistical (sampled atoms).  No counterpart in the
  * reference, which sums +-1 images only (Code/backend_cpu.cpp:51-53). */
 #define DISCO_SURR_IN 14
 #define DISCO_SURR_OUT 3
