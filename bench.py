@@ -138,14 +138,16 @@ def main():
                          "every 8th")
     ap.add_argument("--workload", default="c2",
                     choices=["c2", "c3", "c4", "c5", "copy", "lattice", "writers", "prismatic",
-                             "apply"],
+                             "apply", "datagen"],
                     help="c2 (default): the 4M-atom line-integral sample; c3: loopons; "
-                         "c4: 256 batched samples; c5: the far-shell surrogate; copy: a plain "
+                         "c4: 256 batched samples; c5: images +-4 in cubature form; copy: a plain "
                          "256 MiB device copy (x3), the size reference for HBM byte counters; "
                          "lattice: the 4M-atom lattice generated on the device; writers: the "
                          "4M-atom sample's final positions and .xyz / .lmp files; prismatic: "
                          "slab selection of 20 prismatic loops over the 4M atoms; apply: the "
-                         "far-shell MLP over the 4M atoms and 200 loops")
+                         "far-shell MLP over the 4M atoms and 200 loops; datagen: several "
+                         "256-sample batches with the host producer overlapped")
+    ap.add_argument("--batches", type=int, default=4, help="datagen: batches")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): atoms shard "
