@@ -341,7 +341,8 @@ def _sum_host(ctx, kind, x, y, z, u, loops, box, img, prec, timing):
             raise ValueError("u arrays must be C-contiguous float64 of length n")
     t = _Timing()
     sec = np.zeros(max(1, len(loops)))
-    t.loop_sec = sec.ctypes.data_as(_dp)
+    # per-loop times only when asked for: they run each loop in its own launch
+    t.loop_sec = sec.ctypes.data_as(_dp) if timing is not None else None
     if kind == "line":
         arr, keep = line_structs(loops)
         fn = lib().disco_hip_line_sum

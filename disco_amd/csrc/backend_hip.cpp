@@ -134,8 +134,7 @@ void kroupaGPU(const AtomArrays& atoms, const vector<LoopData>& loops, double dx
             const LoopData& lp = loops[l];
             cout << "  Loop " << l + 1 << "/" << nbLoops << ":  "
                  << (lp.line ? lp.vx.size() : lp.x.size())
-                 << (lp.line ? " segments" : " loopons") << "  [one launch, all loops]  "
-                 << t.loopSec[l] << " s (share of the launch)\n";
+                 << (lp.line ? " segments" : " loopons") << "  " << t.loopSec[l] << " s\n";
         }
     }
     cout << "\nGPU kernel time: " << t.kernels * 1e3 << " ms\n\n";

@@ -742,6 +742,7 @@ struct disco_plan {
     int kofs[5] = {0, 0, 0, 0, 0};   // line plans: loops of kind k are [kofs[k], kofs[k+1])
     bool cubature = false;           // FP32C line plans: the far pass weighs in the cubature
     std::vector<LineLoop> hloops;    // line plans: host copy of the packed loops
+    std::vector<int> src;            // input index of each packed loop
     int split = 1;                   // loopon plans: loop groups per launch (few atoms)
     double* d_split = nullptr;       // their partial sums (3 n split doubles, grow-only)
     size_t cap_split = 0;
@@ -829,14 +830,19 @@ void free_plan(disco_plan* p)
 
 // scratch = nullptr: a split loopon plan keeps its partial sums in its own
 // buffer (allocated, grow-only); otherwise they go to `scratch`, which holds
-// scratch_bytes (a larger need fails before anything is launched).
+// scratch_bytes (a larger need fails before anything is launched).  la, lb:
+// only the packed loops [la, lb) are summed (lb < 0: all of them; a range is
+// summed unsplit).
 int launch(disco_plan* p, int n, const double* x, const double* y, const double* z, double* ux,
            double* uy, double* uz, int accumulate, hipStream_t s, double* scratch = nullptr,
-           size_t scratch_bytes = 0)
+           size_t scratch_bytes = 0, int la = 0, int lb = -1)
 {
     disco_ctx* ctx = p->ctx;
     if (n == 0) return DISCO_OK;
-    const int split = p->split < p->nloops ? p->split : p->nloops;
+    if (lb < 0) lb = p->nloops;
+    const bool ranged = la != 0 || lb != p->nloops;
+    const int nl = lb - la;
+    const int split = ranged ? 1 : (p->split < p->nloops ? p->split : p->nloops);
     const size_t split_need = (size_t)3 * (size_t)n * (size_t)split * sizeof(double);
     if (p->kind == 1 && split > 1 && scratch && split_need > scratch_bytes)
         return fail(ctx, DISCO_ERR_TOO_LARGE,
@@ -844,10 +850,10 @@ int launch(disco_plan* p, int n, const double* x, const double* y, const double*
                         "loop groups) exceed the " + std::to_string(scratch_bytes) +
                         "-byte static scratch");
     const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
-    const bool prof = p->profile && p->nloops > 0;
-    p->profiled_run = false;
+    const bool prof = p->profile && nl > 0 && !ranged;
+    if (!ranged) p->profiled_run = false;
     if (prof) DISCO_HIP_TRY(ctx, hipEventRecord(p->ev[0], s));
-    if (p->nloops == 0) {
+    if (nl <= 0) {
         if (!accumulate) hipLaunchKernelGGL(zero3_kernel, grid, block, 0, s, ux, uy, uz, n);
         DISCO_HIP_TRY(ctx, hipGetLastError());
         return DISCO_OK;
@@ -861,18 +867,19 @@ int launch(disco_plan* p, int n, const double* x, const double* y, const double*
             // skips the others); the first writes u (or adds, as asked).
             // Both always run, so a batch sums in the same order.
             hipLaunchKernelGGL((line_kernel<PASS_FP64, FP64_GLOBAL>), grid, block, 0, s, x, y, z,
-                               ux, uy, uz, n, lp, p->nloops, sf, sd, p->L[0], p->L[1], p->L[2],
+                               ux, uy, uz, n, lp + la, nl, sf, sd, p->L[0], p->L[1], p->L[2],
                                p->img, p->inner, accumulate);
             hipLaunchKernelGGL((line_kernel<PASS_FP64, FP64_FRAME>), grid, block, 0, s, x, y, z,
-                               ux, uy, uz, n, lp, p->nloops, sf, sd, p->L[0], p->L[1], p->L[2],
+                               ux, uy, uz, n, lp + la, nl, sf, sd, p->L[0], p->L[1], p->L[2],
                                p->img, p->inner, 1);
         } else {
             // far pass: one kernel per loop kind present; the first writes u
             // (or adds, as asked), the others add
             int acc = accumulate;
             for (int k = 0; k < 4; k++) {
-                const int l0 = p->kofs[k], cnt = p->kofs[k + 1] - l0;
-                if (cnt == 0) continue;
+                const int l0 = std::max(p->kofs[k], la);
+                const int cnt = std::min(p->kofs[k + 1], lb) - l0;
+                if (cnt <= 0) continue;
                 const bool cub = p->cubature;
                 auto kern = k == LOOP_3D ? line_kernel<PASS_FAR, LOOP_3D>
                           : k == LOOP_FLAT ? (cub ? line_kernel<PASS_FAR, LOOP_FLAT, true>
@@ -886,8 +893,8 @@ int launch(disco_plan* p, int n, const double* x, const double* y, const double*
                 acc = 1;
             }
             if (prof) DISCO_HIP_TRY(ctx, hipEventRecord(p->ev[1], s));
-            hipLaunchKernelGGL(line_near_kernel, grid, block, 0, s, x, y, z, ux, uy, uz, n, lp,
-                               p->nloops, sd, p->L[0], p->L[1], p->L[2], p->img, p->inner);
+            hipLaunchKernelGGL(line_near_kernel, grid, block, 0, s, x, y, z, ux, uy, uz, n,
+                               lp + la, nl, sd, p->L[0], p->L[1], p->L[2], p->img, p->inner);
         }
     } else {
         const LooponLoop* lp = (const LooponLoop*)p->d_loops;
@@ -919,11 +926,11 @@ int launch(disco_plan* p, int n, const double* x, const double* y, const double*
                                ux, uy, uz, accumulate);
         } else if (p->prec == DISCO_PREC_FP64)
             hipLaunchKernelGGL(loopon_kernel<DISCO_PREC_FP64>, grid, block, 0, s, x, y, z, ux, uy,
-                               uz, n, lp, p->nloops, pf, pd, p->L[0], p->L[1], p->L[2], p->img,
+                               uz, n, lp + la, nl, pf, pd, p->L[0], p->L[1], p->L[2], p->img,
                                p->inner, accumulate);
         else
             hipLaunchKernelGGL(loopon_kernel<DISCO_PREC_FP32C>, grid, block, 0, s, x, y, z, ux,
-                               uy, uz, n, lp, p->nloops, pf, pd, p->L[0], p->L[1], p->L[2],
+                               uy, uz, n, lp + la, nl, pf, pd, p->L[0], p->L[1], p->L[2],
                                p->img, p->inner, accumulate);
     }
     DISCO_HIP_TRY(ctx, hipGetLastError());
@@ -949,6 +956,7 @@ int make_line_plan(disco_ctx* ctx, const disco_line_loop* loops, int nloops, dou
     p->nloops = (int)pk.loops.size();
     p->work = pk.segments;
     p->hloops = pk.loops;
+    for (const LineLoop& L : pk.loops) p->src.push_back(L.src);
     std::memcpy(p->kofs, pk.kofs, sizeof(p->kofs));
     const int rc = upload_plan(ctx, p, pk.loops.data(), pk.loops.size() * sizeof(LineLoop),
                                pk.segf.data(), pk.segf.size() * sizeof(SegF), pk.segd.data(),
@@ -972,6 +980,7 @@ int make_loopon_plan(disco_ctx* ctx, const disco_loopon_loop* loops, int nloops,
     p->L[0] = Lx; p->L[1] = Ly; p->L[2] = Lz;
     p->nloops = (int)pk.loops.size();
     p->work = pk.loopons;
+    p->src = pk.src;
     const int rc = upload_plan(ctx, p, pk.loops.data(), pk.loops.size() * sizeof(LooponLoop),
                                pk.pf.data(), pk.pf.size() * sizeof(float4), pk.pd.data(),
                                pk.pd.size() * sizeof(double4));
@@ -1005,13 +1014,53 @@ int host_sum(disco_ctx* ctx, disco_plan* p, const std::vector<double>& work, int
     DISCO_HIP_TRY(ctx, hipMemcpyAsync(dz, z, bytes, hipMemcpyHostToDevice, ctx->stream));
     DISCO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     const double t1 = now_s();
+    // with per-loop times asked for, one launch per loop timed by its own events
+    // (the reference's kroupaGPU launches and times each loop:
+    // Code/backend_cuda.cu:643-700); u then sums loop by loop, as kroupaCPU does
+    const bool per_loop = t && t->loop_sec && p->nloops > 0 && n > 0;
+    std::vector<hipEvent_t> lev;
+    if (per_loop) {
+        lev.assign((size_t)p->nloops + 1, nullptr);
+        for (hipEvent_t& e : lev) {
+            const hipError_t er = hipEventCreate(&e);
+            if (er != hipSuccess) {
+                for (hipEvent_t& f : lev) if (f) (void)hipEventDestroy(f);
+                return fail(ctx, DISCO_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(er));
+            }
+        }
+    }
+    auto destroy = [&]() { for (hipEvent_t& e : lev) if (e) (void)hipEventDestroy(e); };
     DISCO_HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    int rc = launch(p, n, dx, dy, dz, dux, duy, duz, 0, ctx->stream);
-    if (rc != DISCO_OK) return rc;
-    DISCO_HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    DISCO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = DISCO_OK;
+    if (per_loop) {
+        if (hipEventRecord(lev[0], ctx->stream) != hipSuccess) rc = DISCO_ERR_HIP;
+        for (int j = 0; j < p->nloops && rc == DISCO_OK; j++) {
+            rc = launch(p, n, dx, dy, dz, dux, duy, duz, j > 0 ? 1 : 0, ctx->stream, nullptr, 0,
+                        j, j + 1);
+            if (rc == DISCO_OK && hipEventRecord(lev[j + 1], ctx->stream) != hipSuccess)
+                rc = fail(ctx, DISCO_ERR_HIP, "hipEventRecord");
+        }
+    } else {
+        rc = launch(p, n, dx, dy, dz, dux, duy, duz, 0, ctx->stream);
+    }
+    if (rc != DISCO_OK) { destroy(); return rc; }
+    if (hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        destroy();
+        return fail(ctx, DISCO_ERR_HIP, "kernel completion");
+    }
     const double t2 = now_s();
     float ms = 0.f;
+    std::vector<double> lsec;
+    if (per_loop) {
+        lsec.assign((size_t)nloops_in, 0.0);
+        for (int j = 0; j < p->nloops; j++) {
+            float lm = 0.f;
+            if (hipEventElapsedTime(&lm, lev[j], lev[j + 1]) == hipSuccess && p->src[j] < nloops_in)
+                lsec[(size_t)p->src[j]] = lm * 1e-3;
+        }
+    }
+    destroy();
     DISCO_HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     // download into host staging, then accumulate into the caller's u
     std::vector<double> h(3 * (size_t)n);
@@ -1031,12 +1080,8 @@ int host_sum(disco_ctx* ctx, disco_plan* p, const std::vector<double>& work, int
         t->kernels = t2 - t1;
         t->download = t3 - t2;
         t->kernel_ms = ms;
-        if (t->loop_sec) {
-            double tot = 0.0;
-            for (int l = 0; l < nloops_in; l++) tot += work[l];
-            for (int l = 0; l < nloops_in; l++)
-                t->loop_sec[l] = tot > 0 ? t->kernels * work[l] / tot : 0.0;
-        }
+        if (t->loop_sec)
+            for (int l = 0; l < nloops_in; l++) t->loop_sec[l] = per_loop ? lsec[(size_t)l] : 0.0;
     }
     return DISCO_OK;
 }

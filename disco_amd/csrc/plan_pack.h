@@ -33,6 +33,7 @@ struct LinePack {
 
 struct LooponPack {
     std::vector<LooponLoop> loops;
+    std::vector<int> src;          // input index of each packed loop
     std::vector<float4> pf;
     std::vector<double4> pd;
     double loopons = 0;
@@ -425,6 +426,7 @@ inline std::string pack_loopon(const disco_loopon_loop* in, int nloops, double L
         L.flat = flat ? 1 : 0;
         out.loopons += lp.np;
         out.loops.push_back(L);
+        out.src.push_back(l);
     }
     return "";
 }

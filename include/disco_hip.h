@@ -51,8 +51,11 @@ typedef struct disco_ctx disco_ctx;
 typedef struct disco_plan disco_plan;
 
 /* Backend phase times; replaces BackendTiming (Code/backend.hpp:36-39).
- * loop_sec (nloops entries, may be NULL) receives the kernel time apportioned
- * to each loop by its share of the work, since all loops run in one launch. */
+ * loop_sec (nloops entries, may be NULL): when given, the host-buffer calls
+ * launch each loop on its own and store its measured kernel time (HIP events,
+ * seconds), as the reference's kroupaGPU times each loop
+ * (Code/backend_cuda.cu:643-700); a degenerate loop (fewer than 3 distinct
+ * vertices, no loopons) gets 0.  When NULL, every loop runs in one launch. */
 typedef struct {
     double upload;            /* s: allocation + H2D                                  */
     double kernels;           /* s: host wall clock around the kernels (synchronised) */

@@ -54,4 +54,6 @@ def test_kroupa_gpu_driver_mixed_methods(tmp_path, fp32, tol):
     ref = np.stack(oracle.line_sum(x, y, z, [loops[0], loops[2]], c1.box, 1))
     ref += np.stack(oracle.loopon_sum(x, y, z, [loops[1]], c1.box, 1))
     assert np.abs(u - ref).max() < tol
-    assert loop_sec.shape == (3,) and np.all(loop_sec >= 0)
+    assert loop_sec.shape == (3,) and np.all(loop_sec > 0)      # measured per loop
+    upload, kernels, download = data[3 * n + 3:3 * n + 6]
+    assert loop_sec.sum() <= kernels * 1.05 + 1e-3

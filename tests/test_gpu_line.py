@@ -213,3 +213,24 @@ def test_loop_kinds_vs_oracle(hip_ctx, kind):
     print(f"{kind}: fp64 {e64:.2e}  fp32c {e32:.2e}  |u|max {np.abs(ref).max():.2f}")
     assert e64 < TOL64
     assert e32 < TOL32
+
+
+@pytest.mark.parametrize("prec", [B.PREC_FP64, B.PREC_FP32C])
+def test_per_loop_times_are_measured(hip_ctx, prec):
+    """With loop times asked for, the host-buffer sum launches and times each loop
+    on its own (Code/backend_cuda.cu:643-700): every real loop gets a positive
+    time, a degenerate one 0, the times add up to at most the kernel time, and the
+    sum equals the one-launch sum up to the order of the FP64 additions."""
+    s = G.config_c1()
+    idx = np.arange(0, s.n, 37)
+    x, y, z = s.x[idx], s.y[idx], s.z[idx]
+    lp = s.loops[0]
+    dead = B.LoopData(phys=lp.phys, line=True, vx=lp.vx[:2].copy(), vy=lp.vy[:2].copy(),
+                      vz=lp.vz[:2].copy(), cx=lp.cx, cy=lp.cy, cz=lp.cz)
+    loops = [s.loops[0], dead, s.loops[1]]
+    t = B.BackendTiming()
+    timed = np.stack(B.line_sum(hip_ctx, x, y, z, loops, s.box, 1, prec, timing=t))
+    one = np.stack(B.line_sum(hip_ctx, x, y, z, loops, s.box, 1, prec))
+    assert t.loop_sec[0] > 0 and t.loop_sec[2] > 0 and t.loop_sec[1] == 0.0
+    assert sum(t.loop_sec) <= t.kernels * 1.05 + 1e-3
+    assert np.abs(timed - one).max() <= 1e-12 * max(1.0, np.abs(one).max())
