@@ -1033,7 +1033,7 @@ int host_sum(disco_ctx* ctx, disco_plan* p, const std::vector<double>& work, int
     DISCO_HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     int rc = DISCO_OK;
     if (per_loop) {
-        if (hipEventRecord(lev[0], ctx->stream) != hipSuccess) rc = DISCO_ERR_HIP;
+        if (hipEventRecord(lev[0], ctx->stream) != hipSuccess) rc = fail(ctx, DISCO_ERR_HIP, "hipEventRecord");
         for (int j = 0; j < p->nloops && rc == DISCO_OK; j++) {
             rc = launch(p, n, dx, dy, dz, dux, duy, duz, j > 0 ? 1 : 0, ctx->stream, nullptr, 0,
                         j, j + 1);

@@ -1,7 +1,7 @@
 // pair_terms.h -- per-(atom, loop, image) field evaluations of the hot path.
 //
 // Two fields, each in FP64 and FP32:
-//   * the closed-form line integral of a closed planar polyline (the Volterra
+//   * the closed-form line integral of a closed polyline (the Volterra
 //     field: solid-angle term b Omega / 4 pi plus the segment terms J0, K0, K1),
 //     the quantity of Code/barnett.hpp:44-107;
 //   * the Kroupa loopon sum, the quantity of Code/kroupa.hpp:52-73.
@@ -9,9 +9,15 @@
 // The formulas are the reference's, rearranged for a 64-lane VALU:
 //   * Omega: the fan of Van Oosterom-Strackee triangles (c, A, B) gives
 //     Omega = 2 sum_k atan2(num_k, den_k) = 2 arg prod_k (den_k + i num_k).
-//     For a flat contour the signed sum lies in (-2 pi, 2 pi) (its |Omega/2| < pi),
-//     so the principal argument of the running complex product is exact: one
-//     atan2 per loop image instead of one per segment.  num_k = (c - p).(A x B)
+//     For a simple flat contour the signed sum lies in (-2 pi, 2 pi) (its
+//     |Omega/2| < pi), so the principal argument of the running complex
+//     product is exact: one atan2 per loop image instead of one per segment.
+//     Any other contour (off its plane, or winding twice round some point of
+//     it) can reach |Omega| up to 4 pi: its walk counts the running product's
+//     crossings of the negative real axis, each worth 2 pi of the sum (a factor
+//     turns the product by less than pi, so the crossing's side and direction
+//     follow from the signs of Im before and after and of the turn's cross
+//     product -- the reference's per-triangle sum).  num_k = (c - p).(A x B)
 //     with A, B relative to c: the cross product A x B is a per-segment
 //     constant (det[c-p, A-p, B-p] = det[c-p, A-c, B-c]).  Each factor is
 //     scaled by 1/(|c-p||A-p||B-p|) (a positive factor does not move the
@@ -263,6 +269,21 @@ DISCO_HD double two_atanh_d(double z)
     return log_ge1_d<M>((1.0 + z) * M::rcp_d(1.0 - z));
 }
 
+// ------------------------------------------------------------ winding of the product
+// +1 when the running product (X, Y) -> (Xn, Yn) crossed the negative real
+// axis turning counter-clockwise, -1 clockwise, else 0 (Y >= 0 is the upper
+// side, as the principal argument (-pi, pi] takes it).  A turn by less than pi
+// from the upper side to the lower crosses the negative axis exactly when it
+// turns counter-clockwise, X Yn - Xn Y > 0; near that axis (X, Xn < 0) the two
+// products have the same sign, so rounding cannot flip it.
+template <typename R>
+DISCO_HD int winding_step(R X, R Y, R Xn, R Yn)
+{
+    const R cr = X * Yn - Xn * Y;
+    const bool up = Y >= R(0), upn = Yn >= R(0);
+    return (up && !upn && cr > R(0)) ? 1 : ((!up && upn && cr < R(0)) ? -1 : 0);
+}
+
 // ------------------------------------------------------------ line, FP32
 // Field of one loop image in float; q = atom - c - delta (float), s the loop's
 // segments, b the Burgers vector.  Adds into o (float).
@@ -282,6 +303,7 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
     float P = l1 * RA - (qx * ax + qy * ay + qz * az);        // l1 |A-p| + (c-p).(A-p)
 
     float X = 1.0f, Y = 0.0f;
+    int wind = 0;
     float swx = 0.f, swy = 0.f, swz = 0.f;
     float sax = 0.f, say = 0.f, saz = 0.f;
     float stx = 0.f, sty = 0.f, stz = 0.f;
@@ -306,8 +328,10 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
             const float num = -(qx * g.nx + qy * g.ny + qz * g.nz) * sc;
             P = l1 * RB + r1b;
             const float Xn = X * den - Y * num;
-            Y = X * num + Y * den;
+            const float Yn = X * num + Y * den;
+            wind += winding_step(X, Y, Xn, Yn);
             X = Xn;
+            Y = Yn;
 
             // segment terms; K0 = kn / kd, one reciprocal shared with J0's argument
             const float sA = ax * g.tx + ay * g.ty + az * g.tz;
@@ -341,7 +365,7 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
         X = M::ldexp_f(X, -e);
         Y = M::ldexp_f(Y, -e);
     }
-    const float fo = 2.0f * M::atan2_f(Y, X) * (float)kInv4Pi;
+    const float fo = 2.0f * M::atan2_f(Y, X) * (float)kInv4Pi + (float)wind;
     ox += bx * fo + (swx - sax - stx);
     oy += by * fo + (swy - say - sty);
     oz += bz * fo + (swz - saz - stz);
@@ -551,6 +575,7 @@ DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __rest
     double P = l1 * RA + (l1sq - (qx * last.bx + qy * last.by + qz * last.bz));
 
     double X = 1.0, Y = 0.0;
+    int wind = 0;
     double swx = 0., swy = 0., swz = 0., sax = 0., say = 0., saz = 0., stx = 0., sty = 0., stz = 0.;
     for (int k = 0; k < nv; k++) {
         const SegD& g = s[k];
@@ -568,8 +593,10 @@ DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __rest
         den *= sc;
         num *= sc;
         const double Xn = X * den - Y * num;
-        Y = X * num + Y * den;
+        const double Yn = X * num + Y * den;
+        wind += winding_step(X, Y, Xn, Yn);
         X = Xn;
+        Y = Yn;
         if ((k & 63) == 63) {
             const int e = M::exponent_d(M::fmax_d(M::fabs_d(X), M::fabs_d(Y)));
             X = M::ldexp_d(X, -e);
@@ -600,7 +627,7 @@ DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __rest
         ax = px; ay = py; az = pz;
         RA = RB; rA = rB;
     }
-    const double fo = 2.0 * atan2_own_d<M>(Y, X) * kInv4Pi;
+    const double fo = 2.0 * atan2_own_d<M>(Y, X) * kInv4Pi + (double)wind;
     ox += bx * fo + (swx - sax - stx);
     oy += by * fo + (swy - say - sty);
     oz += bz * fo + (swz - saz - stz);

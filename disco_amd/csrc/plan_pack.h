@@ -40,6 +40,74 @@ struct LooponPack {
     std::vector<double> work;
 };
 
+// A normal for a contour with no vector area: the plane of the vertex
+// farthest from V[0] and the one that spans the largest parallelogram with it
+// (for collinear vertices, any direction normal to their line).
+inline void fallback_normal(const std::vector<double>& ax, const std::vector<double>& ay,
+                            const std::vector<double>& az, double& Nx, double& Ny, double& Nz)
+{
+    const int nv = (int)ax.size();
+    int kf = 0;
+    double best = -1.0;
+    for (int k = 0; k < nv; k++) {
+        const double d2 = (ax[k] - ax[0]) * (ax[k] - ax[0]) + (ay[k] - ay[0]) * (ay[k] - ay[0]) +
+                          (az[k] - az[0]) * (az[k] - az[0]);
+        if (d2 > best) { best = d2; kf = k; }
+    }
+    const double ux = ax[kf] - ax[0], uy = ay[kf] - ay[0], uz = az[kf] - az[0];
+    best = 0.0;
+    Nx = Ny = Nz = 0.0;
+    for (int k = 0; k < nv; k++) {
+        const double wx = ax[k] - ax[0], wy = ay[k] - ay[0], wz = az[k] - az[0];
+        const double cx = uy * wz - uz * wy, cy = uz * wx - ux * wz, cz = ux * wy - uy * wx;
+        const double c2 = cx * cx + cy * cy + cz * cz;
+        if (c2 > best) { best = c2; Nx = cx; Ny = cy; Nz = cz; }
+    }
+    if (!(best > 1e-24 * (ux * ux + uy * uy + uz * uz) * (ux * ux + uy * uy + uz * uz))) {
+        // collinear: u x (the axis along u's smallest component)
+        const double fx = std::fabs(ux), fy = std::fabs(uy), fz = std::fabs(uz);
+        const double ex = (fx <= fy && fx <= fz) ? 1.0 : 0.0;
+        const double ey = (ex == 0.0 && fy <= fz) ? 1.0 : 0.0;
+        const double ez = (ex == 0.0 && ey == 0.0) ? 1.0 : 0.0;
+        Nx = uy * ez - uz * ey; Ny = uz * ex - ux * ez; Nz = ux * ey - uy * ex;
+    }
+}
+
+// True when the closed polygon (p1[k], p2[k]) in its plane is simple, so that
+// its winding number is 0 or +-1 about every point: star-shaped about the
+// origin (the apex: every fan triangle turns the same way, once round), or
+// else no two non-adjacent edges touch.  Ties count as touching, so a
+// degenerate polygon is called not simple (and summed by the general walk).
+inline bool simple_polygon(const std::vector<double>& p1, const std::vector<double>& p2)
+{
+    const int n = (int)p1.size();
+    if (n <= 3) return true;
+    bool pos = true, neg = true;
+    double turn = 0.0;
+    for (int k = 0; k < n; k++) {
+        const int j = (k + n - 1) % n;
+        const double cr = p1[j] * p2[k] - p2[j] * p1[k];
+        pos = pos && cr > 0.0;
+        neg = neg && cr < 0.0;
+        turn += std::atan2(cr, p1[j] * p1[k] + p2[j] * p2[k]);
+    }
+    if ((pos || neg) && std::fabs(turn) < 3.0 * 3.14159265358979323846) return true;
+    auto orient = [&](int a, int b, int c) {
+        return (p1[b] - p1[a]) * (p2[c] - p2[a]) - (p2[b] - p2[a]) * (p1[c] - p1[a]);
+    };
+    auto straddle = [](double u, double v) { return (u <= 0.0 && v >= 0.0) || (u >= 0.0 && v <= 0.0); };
+    // edge k runs V[k-1] -> V[k]
+    for (int a = 0; a < n; a++)
+        for (int b = a + 2; b < n; b++) {
+            if (a == 0 && b == n - 1) continue;   // adjacent through V[n-1] -> V[0]
+            const int a0 = (a + n - 1) % n, b0 = b - 1;
+            if (straddle(orient(a0, a, b0), orient(a0, a, b)) &&
+                straddle(orient(b0, b, a0), orient(b0, b, a)))
+                return false;
+        }
+    return true;
+}
+
 // Height of the near slab: a lane whose loop image lies within this distance
 // of the loop's plane (and inside its radius + this margin) is evaluated in
 // FP64 in the FP32C mode.
@@ -173,8 +241,18 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, d
             Ny += az[j] * ax[k] - ax[j] * az[k];
             Nz += ax[j] * ay[k] - ay[j] * ax[k];
         }
-        const double Nn = std::sqrt(Nx * Nx + Ny * Ny + Nz * Nz);
-        if (!(Nn > 0.0)) return "polyline with zero vector area";
+        double Nn = std::sqrt(Nx * Nx + Ny * Ny + Nz * Nz);
+        double vmax2 = 0.0;
+        for (int k = 0; k < nv; k++)
+            vmax2 = std::fmax(vmax2, (ax[k] - ax[0]) * (ax[k] - ax[0]) +
+                                         (ay[k] - ay[0]) * (ay[k] - ay[0]) +
+                                         (az[k] - az[0]) * (az[k] - az[0]));
+        if (!(Nn > 1e-12 * vmax2)) {
+            // no plane from the vector area (a contour whose lobes cancel):
+            // any normal serves, the loop is summed in global axes
+            fallback_normal(ax, ay, az, Nx, Ny, Nz);
+            Nn = std::sqrt(Nx * Nx + Ny * Ny + Nz * Nz);
+        }
         const double nx = Nx / Nn, ny = Ny / Nn, nz = Nz / Nn;
         double hmax = 0.0, rmax = 0.0;
         for (int k = 0; k < nv; k++) {
@@ -183,8 +261,6 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, d
             hmax = std::fmax(hmax, std::fabs(h));
             rmax = std::fmax(rmax, std::sqrt(std::fmax(r2, 0.0)));
         }
-        if (hmax > 1e-6 * std::fmax(rmax, 1.0))
-            return "polyline is not planar (the solid angle uses a flat cut)";
 
         LineLoop L;
         std::memset(&L, 0, sizeof(L));
@@ -226,7 +302,10 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, d
             p1[k] = ax[k] * L.e1x + ay[k] * L.e1y + az[k] * L.e1z;
             p2[k] = ax[k] * L.e2x + ay[k] * L.e2y + az[k] * L.e2z;
         }
-        bool flat = hmax <= res;
+        // the frame forms take the solid angle's principal argument, exact
+        // for a simple polygon only
+        const bool simple = simple_polygon(p1, p2);
+        bool flat = hmax <= res && simple;
         for (int k = 0; k < nv && flat; k++) {
             const int j = (k + nv - 1) % nv;
             flat = std::hypot(p1[k] - p1[j], p2[k] - p2[j]) > res;
@@ -235,7 +314,7 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, d
         // plane to double resolution (every DISCO shape: heights ~1e-16 of its
         // size); otherwise Seg<double> in global axes
         const double res64 = std::ldexp(std::fmax(rmax, 1.0), -48);
-        bool flat64 = hmax <= res64;
+        bool flat64 = hmax <= res64 && simple;
         for (int k = 0; k < nv && flat64; k++) {
             const int j = (k + nv - 1) % nv;
             flat64 = std::hypot(p1[k] - p1[j], p2[k] - p2[j]) > res64;

@@ -187,10 +187,10 @@ def test_invalid_arguments_raise(hip_ctx):
         B.line_sum(hip_ctx, x, x, x, s.loops, s.box, 1, prec=7)
     with pytest.raises(B.DiscoError):
         B.line_sum(hip_ctx, x, x, x, s.loops, s.box, 99, B.PREC_FP64)
-    bent = B.LoopData(phys=s.loops[0].phys, line=True, vx=np.array([0., 10, 10, 0]),
-                      vy=np.array([0., 0, 10, 10]), vz=np.array([0., 0, 0, 5]))
+    bad = B.LoopData(phys=s.loops[0].phys, line=True, vx=np.array([0., 10, np.nan, 0]),
+                     vy=np.array([0., 0, 10, 10]), vz=np.array([0., 0, 0, 5]))
     with pytest.raises(B.DiscoError):
-        B.line_sum(hip_ctx, x, x, x, [bent], s.box, 1, B.PREC_FP64)
+        B.line_sum(hip_ctx, x, x, x, [bad], s.box, 1, B.PREC_FP64)
 
 
 @pytest.mark.parametrize("kind", ["glide", "prism", "mixed", "bent", "all"])
