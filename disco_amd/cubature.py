@@ -21,6 +21,10 @@ hexagons).  The cubature form is
   them by 6-node Lagrange (``disco_hip_grid_interp``); so are the +-1 images so
   far from every atom that the cubature alone is accurate at all of them and
   a finer grid interpolates their field to a stated tolerance (``far_images``).
+  A loop whose cut is not a flat region about its apex (off its plane, lobes
+  of opposite sense, ...) is summed there as Radon's rule on each of its fan
+  triangles instead (``surrogate.fan_loopons``), and its +-1 images stay in
+  the line plan.
 
 Its error against the plain sum is measured, never assumed: the bench and the
 tests compare it with the FP64 plain sum.
@@ -144,23 +148,28 @@ class CubatureSum:
             hn = float(near_h) if near_h else min(box) / (32.0 if order == 6 else 64.0)
             lo = [float(t.min()) for t in (x, y, z)]
             hi = [float(t.max()) for t in (x, y, z)]
+            info = B.line_cubature(loops)
+            for l, lp in enumerate(loops):      # the grid sums the ring rule's cubature
+                if info[l, 0] and not S.ring_rule_ok(lp):
+                    info[l, 0] = 0.0
             masks, far = far_images(loops, box, lo, hi, hn, float(near_tol),
-                                    max(float(k_exact), float(k_rule or 0.0)), order=order)
+                                    max(float(k_exact), float(k_rule or 0.0)), info=info,
+                                    order=order)
             if far:
                 ctx.check(B.lib().disco_hip_plan_set_image_skip(
                     self.near._h, masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
                     len(loops)))
-                base = S.base_loopons(loops, "rule9")
+                base = S.rule_loopons_per_loop(loops, ring=8)
                 copies = []
                 for l, off in far:
-                    for part in base[2 * l:2 * l + 2]:
+                    for part in base[l]:
                         copies.append(B.LoopData(phys=part.phys, line=False, x=part.x + off[0],
                                                  y=part.y + off[1], z=part.z + off[2]))
                 parts.setdefault(hn, []).append((copies, 0, -1))
                 self.skipped = len(far)
         if self.img > 1:
-            base = S.base_loopons(loops, "rule9") if base is None else base
-            parts.setdefault(hf, []).append((base, self.img, 1))
+            base = S.rule_loopons_per_loop(loops, ring=8) if base is None else base
+            parts.setdefault(hf, []).append(([p for ps in base for p in ps], self.img, 1))
         for h, pl in parts.items():
             self.grids.append(_Grid(ctx, x, y, z, box, h, pl, grid_groups, order))
 

@@ -168,11 +168,82 @@ def rule7_loopons(lp: B.LoopData):
     return rule7_loopons_many([lp])
 
 
+# Radon's 7-point degree-5 rule on a triangle, exact: barycentric classes
+# (1/3, 1/3, 1/3), the 3 permutations of (a1, b1, b1), of (a2, b2, b2)
+_S15 = float(np.sqrt(15.0))
+_RADON = ((9 / 40, ((1 / 3, 1 / 3, 1 / 3),)),
+          ((155 + _S15) / 1200, tuple(np.roll([(9 - 2 * _S15) / 21, (6 + _S15) / 21,
+                                               (6 + _S15) / 21], k) for k in range(3))),
+          ((155 - _S15) / 1200, tuple(np.roll([(9 + 2 * _S15) / 21, (6 - _S15) / 21,
+                                               (6 - _S15) / 21], k) for k in range(3))))
+
+
+def ring_rule_ok(lp: B.LoopData, tol=1e-7) -> bool:
+    """The centred ring rules stand for the cut only when it is a flat region
+    about the apex: every vertex within tol R of the plane of the vector area,
+    every fan triangle (c, V_k, V_k+1) turning the way that area does
+    (star-shaped about c)."""
+    c = np.array([lp.cx, lp.cy, lp.cz])
+    V = np.stack([lp.vx, lp.vy, lp.vz], 1) - c
+    X = _cross_rows(V, np.roll(V, -1, axis=0))
+    N = X.sum(0)
+    nN = float(np.linalg.norm(N))
+    if not nN > 0.0:
+        return False
+    n = N / nN
+    R = float(np.sqrt((V * V).sum(1)).max())
+    return bool(np.abs(V @ n).max() <= tol * R and np.all(X @ n > 0.0))
+
+
+def fan_loopons(lp: B.LoopData):
+    """The cut of any closed polyline -- the fan of triangles (c, V_k, V_k+1)
+    that Code/barnett.hpp:44-107 takes -- as Radon's 7-point rule on each fan
+    triangle with that triangle's own normal (V_k - c) x (V_k+1 - c) / |...|
+    and area weights: three loopon LoopData per triangle, one per weight class.
+    Exact through degree 5 on every triangle, for loops off their plane, with
+    lobes of opposite sense or winding more than once."""
+    c = np.array([lp.cx, lp.cy, lp.cz])
+    V = np.stack([lp.vx, lp.vy, lp.vz], 1) - c
+    V1 = np.roll(V, -1, axis=0)
+    X = _cross_rows(V, V1)
+    b = np.array(lp.burgers())
+    bm = float(np.linalg.norm(b))
+    m = b / bm
+    out = []
+    for k in range(len(V)):
+        two_a = float(np.linalg.norm(X[k]))
+        if not two_a > 0.0:
+            continue
+        nk = X[k] / two_a
+        for w, bary in _RADON:
+            P = np.array([c + bb[1] * V[k] + bb[2] * V1[k] for bb in bary])
+            out.append(B.LoopData(phys=(bm, bm, bm, *m, *nk, w * 0.5 * two_a), line=False,
+                                  x=P[:, 0].copy(), y=P[:, 1].copy(), z=P[:, 2].copy()))
+    return out
+
+
+def rule_loopons_per_loop(loops, ring=8):
+    """Per loop, the loopons of its cut: [centre, ring] of the centred ring
+    rule (rule7_loopons_many) where ring_rule_ok, else fan_loopons."""
+    ok = [ring_rule_ok(lp) for lp in loops]
+    idx = [k for k in range(len(loops)) if ok[k]]
+    flat = rule7_loopons_many([loops[k] for k in idx], ring) if idx else []
+    out = [None] * len(loops)
+    for j, k in enumerate(idx):
+        out[k] = flat[2 * j:2 * j + 2]
+    for k in range(len(loops)):
+        if not ok[k]:
+            out[k] = fan_loopons(loops[k])
+    return out
+
+
 def rule7_loopons_many(loops, ring=6):
     """rule7_loopons of many loops, computed together for the loops of each
     vertex count: [centre, ring] per loop, in the loops' order.  ring = 8 gives
     the 9-point rule (w0 + 8 w1 = A, 8 w1 rho^2 = I2, 8 w1 rho^4 = I4; ring
-    angles j pi/4), whose first angular error is at degree 8 instead of 6."""
+    angles j pi/4), whose first angular error is at degree 8 instead of 6.
+    For flat cuts about the apex (ring_rule_ok); base_loopons and
+    rule_loopons_per_loop take fan_loopons for the others."""
     out = [None] * (2 * len(loops))
     by_nv = {}
     for i, lp in enumerate(loops):
@@ -224,10 +295,9 @@ def rule7_loopons_many(loops, ring=6):
 def base_loopons(loops, kind: str):
     if kind == "dipole":
         return [dipole_loopon(l) for l in loops]
-    if kind == "rule7":
-        return rule7_loopons_many(loops)
-    if kind == "rule9":
-        return rule7_loopons_many(loops, ring=8)
+    if kind in ("rule7", "rule9"):
+        per = rule_loopons_per_loop(loops, ring=6 if kind == "rule7" else 8)
+        return [part for parts in per for part in parts]
     raise ValueError(f"unknown baseline {kind!r}")
 
 

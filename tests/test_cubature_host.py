@@ -102,3 +102,51 @@ def test_far_image_selection():
     # loops the rule does not represent are never taken out
     m2, f2 = C.far_images([_rect(5, 3)], box, lo, hi, h, tol, k)
     assert int(m2[0]) == 0 and not f2
+
+
+# ------------------------------------------------------------ cuts that are not flat discs
+def _far_points(c, R, rng, dist, count=40):
+    d = rng.normal(size=(count, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return c + dist * R * d
+
+
+@pytest.mark.parametrize("name", ["saddle", "crown", "cone", "figure8", "figure8_3d", "apex_off"])
+def test_fan_loopons_carry_the_far_field_of_any_polyline(name):
+    """Radon's rule on every fan triangle, each with its own normal, against the
+    oracle's line integral (Code/barnett.hpp:44-107) of the same polyline: the
+    error falls as (R/r)^6 of |b| or faster (degree 5 on each triangle), and
+    the centred ring rule would not stand for these cuts (ring_rule_ok)."""
+    import oracle
+    import polylines as PL
+    rng = np.random.default_rng(PL.SHAPES.index(name) + 80)
+    V, apex, n, _, _, R = PL.shape(name, rng, nseg=24)
+    b = PL.burgers(rng)
+    lp = PL.loop_data(V, apex, b, n)
+    assert not S.ring_rule_ok(lp)
+    fan = S.fan_loopons(lp)
+    Rf = float(np.linalg.norm(V - apex, axis=1).max())
+    box = (1e6, 1e6, 1e6)
+    for dist in (6.0, 12.0):
+        P = _far_points(V.mean(0), Rf, rng, dist)
+        x, y, z = (P[:, k].copy() for k in range(3))
+        ul = np.stack(oracle.line_sum(x, y, z, [lp], box, 0))
+        uf = np.stack(oracle.loopon_sum(x, y, z, fan, box, 0))
+        err = np.abs(ul - uf).max()
+        assert err < 0.05 * np.linalg.norm(b) * dist ** -6, (dist, err)
+
+
+def test_ring_rule_kept_for_flat_cuts_about_the_apex():
+    """Every DISCO loop (flat, star-shaped about its centre) keeps the ring rule;
+    per-loop lists hold [centre, ring] for those and the fan rule for the rest."""
+    import polylines as PL
+    s = G.config_c2(seed=5, nloops=10)
+    assert all(S.ring_rule_ok(lp) for lp in s.loops)
+    rng = np.random.default_rng(3)
+    odd = PL.loops(rng, ("saddle", "figure8"))
+    per = S.rule_loopons_per_loop(s.loops[:3] + odd, ring=8)
+    assert [len(p) for p in per[:3]] == [2, 2, 2]
+    assert len(per[3]) == 3 * 40 and len(per[4]) == 3 * 40
+    flat = S.rule7_loopons_many(s.loops[:3], ring=8)
+    for k in range(3):
+        assert per[k][0].phys == flat[2 * k].phys and np.array_equal(per[k][1].x, flat[2 * k + 1].x)

@@ -177,3 +177,26 @@ def test_cubature_off_is_the_plain_fp32c_pass_and_errors(hip_ctx):
     with pytest.raises(B.DiscoError):
         p64.set_cubature(5.0)
     p64.free()
+
+
+def test_polylines_off_the_plane_in_cubature_form(hip_ctx):
+    """Closed polylines that are not flat discs (saddle, cone, figure eights,
+    star, apex off the plane: polylines.py) beside circles, images +-3: their
+    +-1 images stay exact in the line plan and their far shell is Radon's rule
+    on each fan triangle (surrogate.fan_loopons); within 1e-5 A of the FP64
+    plain sum."""
+    import torch
+    import polylines as PL
+    rng = np.random.default_rng(23)
+    loops = PL.loops(rng) + kind_loops("mixed", rng, 2)
+    x, y, z = _dev(*PL.atoms(loops, rng, nrand=3000, per_loop=100))
+    ref = _plain64(hip_ctx, x, y, z, loops, PL.BOX, 3)
+    cs = C.CubatureSum(hip_ctx, x, y, z, loops, PL.BOX, 3, K_EXACT, near_tol=NEAR_TOL)
+    assert cs.eligible == 2                       # the circles only
+    u = [torch.zeros_like(x) for _ in range(3)]
+    cs.run(x, y, z, *u)
+    torch.cuda.synchronize()
+    cs.free()
+    e = _err(u, ref)
+    print(f"polylines in cubature form, +-3: {e:.3e} A")
+    assert e < TOL
