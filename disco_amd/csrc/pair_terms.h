@@ -143,9 +143,10 @@ struct alignas(64) SegP {
     float b1, b2;       // B - c in the plane
     float t1, t2;       // unit tangent
     float L;            // |B - A|
-    float n3;           // ((A - c) x (B - c)).n = A1 B2 - A2 B1
+    float iL;           // 1 / L
     float w1, w2, w3;   // b x t in the frame
-    float pad[7];
+    float e1, e2;       // B - A
+    float pad[5];
 };
 static_assert(sizeof(SegP) == sizeof(Seg<float>), "one FP32 segment slot per segment");
 
@@ -155,9 +156,10 @@ struct alignas(128) SegPD {
     double b1, b2;      // B - c in the plane
     double t1, t2;      // unit tangent
     double L;           // |B - A|
-    double n3;          // A1 B2 - A2 B1
+    double iL;          // 1 / L
     double w1, w2, w3;  // b x t in the frame
-    double pad[7];
+    double e1, e2;      // B - A
+    double pad[5];
 };
 static_assert(sizeof(SegPD) == sizeof(Seg<double>), "one FP64 segment slot per segment");
 
@@ -375,51 +377,60 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
 // The same algebra in the loop's frame, where the lane sits at q = (q1, q2, h)
 // and every vertex at height 0: B - p = (B1 - q1, B2 - q2, -h), so dot
 // products with positions and tangents take two products plus a per-image
-// constant, and num = -(q.N) = -h n3.  The solid-angle terms carry 1/l1:
-// P' = P/l1, (c-p).(B-p)/l1 = h^2/l1 - (q/l1).(B-p), so den/l1 needs no l1.
+// constant.  The solid angle of a flat contour is that of its region whatever
+// the in-plane apex of the fan, so the walk fans it from the lane's foot
+// F = (q1, q2, 0) instead of c: with c - p = (0, 0, -h), l1 = |h|, Van
+// Oosterom's factor of triangle (F, A, B) is |h| [RA RB + |h| (RA + RB) + a.b
+// - i sgn(h) (a x b).n] (a = A - p, b = B - p), so per segment, scaled by
+// 1 / (|h| RA RB), the walk multiplies g = 1 + (|h| RS + a.b) / (RA RB)
+// + i (a x b).n / (RA RB) (|g| <= 4) and Omega = -sgn(h) 2 arg prod g, with
+// (a x b).n = a x (B - A) from the per-segment edge B - A: free of the
+// cancellation of a1 b2 - a2 b1 (terms |q|^2 for a result |q| L) for a lane
+// far from the loop.
+// For h = 0 this gives 0, the field outside the cut; a lane on the cut itself
+// is in the near slab, which the FP64 pass evaluates.
 struct FlatWalk {
-    float a1, a2, RA, rA, P;                 // A - p in the plane, |A-p|, 1/|A-p|, P/l1
+    float a1, a2, RA, rA;                    // A - p in the plane, |A-p|, 1/|A-p|
     float X, Y;                              // running solid-angle product
     float sw1, sw2, sw3;                     // sum J0 w
     float x1, x2, sal;                       // sum (al a + be t) in the plane, sum al (a3 = -h)
 };
 
 template <class M, int KIND>
-DISCO_HD void flat_step(const SegP& g, float q1, float q2, float h2, float u1, float u2,
-                        float uh2, float nh, float mh, FlatWalk& w)
+DISCO_HD void flat_step(const SegP& g, float q1, float q2, float h2, float ah, float mh,
+                        FlatWalk& w)
 {
     const float p1 = g.b1 - q1, p2 = g.b2 - q2;                 // B - p, height -h
     const float lb2 = p1 * p1 + (p2 * p2 + h2);
     const float rB = M::rsqrt_f(lb2);
     const float RB = lb2 * rB;
     const float rArB = w.rA * rB;
+    const float RS = w.RA + RB;
 
-    const float r1b = (uh2 - u1 * p1) - u2 * p2;                // (c-p).(B-p) / l1
     const float ab = w.a1 * p1 + (w.a2 * p2 + h2);              // (A-p).(B-p)
-    const float den = (RB * w.P + (w.RA * r1b + ab)) * rArB;
-    const float num = (g.n3 * nh) * rArB;
-    w.P = RB + r1b;
+    const float den = (ah * RS + ab) * rArB + 1.0f;
+    const float num = (w.a1 * g.e2 - w.a2 * g.e1) * rArB;
     const float Xn = w.X * den - w.Y * num;
     w.Y = w.X * num + w.Y * den;
     w.X = Xn;
 
+    // K0 = L kn / kd, J0 = 2 atanh(L / RS): one reciprocal of RS kd serves both
     const float sA = w.a1 * g.t1 + w.a2 * g.t2;
     const float sB = sA + g.L;
     const float s2 = sA + sB;
-    const float RS = w.RA + RB;
-    float kn = g.L * s2 * rArB;
+    float kn = s2 * rArB;
     float kd = sB * w.RA + sA * RB;
     const bool mixed = M::fabs_f(s2) < g.L;                     // sA < 0 < sB
     if (__builtin_expect(M::any(mixed), 0)) {                   // a branch, not selects
         const float d1 = w.a1 - sA * g.t1, d2 = w.a2 - sA * g.t2;
         const float dd = d1 * d1 + (d2 * d2 + h2);
-        const float knm = sB * rB - sA * w.rA;
+        const float knm = (sB * rB - sA * w.rA) * g.iL;
         kd = mixed ? dd : kd;
         kn = mixed ? knm : kn;
     }
-    const float rr = M::rcp_f(RS * kd);
-    const float J0 = two_atanh_f<M>(g.L * kd * rr);
-    const float K0 = kn * RS * rr;
+    const float Lrr = g.L * M::rcp_f(RS * kd);
+    const float J0 = two_atanh_f<M>(kd * Lrr);
+    const float K0 = kn * (RS * Lrr);
     const float K1 = w.rA - rB;
     // d.w = a.w with a = (a1, a2, -h); LOOP_GLIDE: a.w = -h w3, the factor -h
     // is applied once after the walk
@@ -445,10 +456,7 @@ template <class M, int KIND>
 DISCO_HD void line_image_flat_f32(float q1, float q2, float h, const SegP* __restrict__ s, int nv,
                                   float& fo, float& o1, float& o2, float& o3)
 {
-    const float h2 = h * h;
-    const float l1sq = q1 * q1 + (q2 * q2 + h2);
-    const float rl1 = M::rsqrt_f(l1sq);
-    const float u1 = q1 * rl1, u2 = q2 * rl1, uh2 = h2 * rl1, nh = -h * rl1, mh = -h;
+    const float h2 = h * h, ah = M::fabs_f(h), mh = -h;
 
     FlatWalk w;
     const SegP& last = s[nv - 1];
@@ -457,7 +465,6 @@ DISCO_HD void line_image_flat_f32(float q1, float q2, float h, const SegP* __res
     const float la2 = w.a1 * w.a1 + (w.a2 * w.a2 + h2);
     w.rA = M::rsqrt_f(la2);
     w.RA = la2 * w.rA;
-    w.P = w.RA + (uh2 - (u1 * w.a1 + u2 * w.a2));              // (l1 |A-p| + (c-p).(A-p)) / l1
     w.X = 1.0f; w.Y = 0.0f;
     w.sw1 = w.sw2 = w.sw3 = 0.0f;
     w.x1 = w.x2 = w.sal = 0.0f;
@@ -467,15 +474,16 @@ DISCO_HD void line_image_flat_f32(float q1, float q2, float h, const SegP* __res
         const int k1 = nv < k0 + 16 ? nv : k0 + 16;
         int k = k0;
         for (; k + 1 < k1; k += 2) {
-            flat_step<M, KIND>(s[k], q1, q2, h2, u1, u2, uh2, nh, mh, w);
-            flat_step<M, KIND>(s[k + 1], q1, q2, h2, u1, u2, uh2, nh, mh, w);
+            flat_step<M, KIND>(s[k], q1, q2, h2, ah, mh, w);
+            flat_step<M, KIND>(s[k + 1], q1, q2, h2, ah, mh, w);
         }
-        for (; k < k1; k++) flat_step<M, KIND>(s[k], q1, q2, h2, u1, u2, uh2, nh, mh, w);
+        for (; k < k1; k++) flat_step<M, KIND>(s[k], q1, q2, h2, ah, mh, w);
         const int e = M::exponent_f(M::fmax_f(M::fabs_f(w.X), M::fabs_f(w.Y)));
         w.X = M::ldexp_f(w.X, -e);
         w.Y = M::ldexp_f(w.Y, -e);
     }
-    fo = 2.0f * M::atan2_f(w.Y, w.X) * (float)kInv4Pi;
+    const float om = 2.0f * M::atan2_f(w.Y, w.X) * (float)kInv4Pi;
+    fo = h > 0.0f ? -om : om;
     float x1 = w.x1, x2 = w.x2, x3 = mh * w.sal;
     if (KIND == LOOP_GLIDE) { x1 *= mh; x2 *= mh; x3 *= mh; }
     o1 = (float)kCJ * w.sw1 - (float)kC3 * x1;
@@ -635,12 +643,12 @@ DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __rest
 
 // ------------------------------------------------------------ line, FP64, loop frame
 // line_image_flat_f32's walk in FP64 for a loop flat to double resolution:
-// the atom at (q1, q2, h) in the loop frame, segments as SegPD; the general
-// form (any b).  Reciprocal square roots and reciprocals from the hardware
+// the atom at (q1, q2, h) in the loop frame, segments as SegPD, the fan from
+// the lane's foot; the general form (any b).  Reciprocal square roots and reciprocals from the hardware
 // estimates refined by Newton steps; 2 atanh from its series unless a lane of
 // the wave needs the log form.
 struct FlatWalkD {
-    double a1, a2, RA, rA, P;
+    double a1, a2, RA, rA;
     double X, Y;
     double sw1, sw2, sw3;
     double x1, x2, sal;
@@ -677,20 +685,19 @@ DISCO_HD double two_atanh_vote_d(double z)
 // GLIDE: b in the loop plane, so b x t is along n (w1 = w2 = 0) and a.w =
 // -h w3: the factor -h is applied once after the walk, as in the FP32 form.
 template <class M, bool GLIDE>
-DISCO_HD void flat_step_d(const SegPD& g, double q1, double q2, double h2, double u1, double u2,
-                          double uh2, double nh, double mh, FlatWalkD& w)
+DISCO_HD void flat_step_d(const SegPD& g, double q1, double q2, double h2, double ah,
+                          double mh, FlatWalkD& w)
 {
     const double p1 = g.b1 - q1, p2 = g.b2 - q2;
     const double lb2 = p1 * p1 + (p2 * p2 + h2);
     const double rB = M::rsqrt_d1(lb2);
     const double RB = lb2 * rB;
     const double rArB = w.rA * rB;
+    const double RS = w.RA + RB;
 
-    const double r1b = (uh2 - u1 * p1) - u2 * p2;
     const double ab = w.a1 * p1 + (w.a2 * p2 + h2);
-    const double den = (RB * w.P + (w.RA * r1b + ab)) * rArB;
-    const double num = (g.n3 * nh) * rArB;
-    w.P = RB + r1b;
+    const double den = (ah * RS + ab) * rArB + 1.0;
+    const double num = (w.a1 * g.e2 - w.a2 * g.e1) * rArB;
     const double Xn = w.X * den - w.Y * num;
     w.Y = w.X * num + w.Y * den;
     w.X = Xn;
@@ -698,20 +705,19 @@ DISCO_HD void flat_step_d(const SegPD& g, double q1, double q2, double h2, doubl
     const double sA = w.a1 * g.t1 + w.a2 * g.t2;
     const double sB = sA + g.L;
     const double s2 = sA + sB;
-    const double RS = w.RA + RB;
-    double kn = g.L * s2 * rArB;
+    double kn = s2 * rArB;
     double kd = sB * w.RA + sA * RB;
     const bool mixed = M::fabs_d(s2) < g.L;                     // sA < 0 < sB
     if (__builtin_expect(M::any(mixed), 0)) {
         const double d1 = w.a1 - sA * g.t1, d2 = w.a2 - sA * g.t2;
         const double dd = d1 * d1 + (d2 * d2 + h2);
-        const double knm = sB * rB - sA * w.rA;
+        const double knm = (sB * rB - sA * w.rA) * g.iL;
         kd = mixed ? dd : kd;
         kn = mixed ? knm : kn;
     }
-    const double rr = M::rcp_d1(RS * kd);
-    const double J0 = two_atanh_vote_d<M>(g.L * kd * rr);
-    const double K0 = kn * RS * rr;
+    const double Lrr = g.L * M::rcp_d1(RS * kd);
+    const double J0 = two_atanh_vote_d<M>(kd * Lrr);
+    const double K0 = kn * (RS * Lrr);
     const double K1 = w.rA - rB;
     const double dw = GLIDE ? g.w3 : w.a1 * g.w1 + (w.a2 * g.w2 + mh * g.w3);
     const double al = dw * K0;
@@ -735,9 +741,7 @@ template <class M, bool GLIDE = false>
 DISCO_HD void line_image_flat_f64(double q1, double q2, double h, const SegPD* __restrict__ s,
                                   int nv, double& fo, double& o1, double& o2, double& o3)
 {
-    const double h2 = h * h;
-    const double rl1 = M::rsqrt_d1(q1 * q1 + (q2 * q2 + h2));
-    const double u1 = q1 * rl1, u2 = q2 * rl1, uh2 = h2 * rl1, nh = -h * rl1, mh = -h;
+    const double h2 = h * h, ah = M::fabs_d(h), mh = -h;
 
     FlatWalkD w;
     const SegPD& last = s[nv - 1];
@@ -746,7 +750,6 @@ DISCO_HD void line_image_flat_f64(double q1, double q2, double h, const SegPD* _
     const double la2 = w.a1 * w.a1 + (w.a2 * w.a2 + h2);
     w.rA = M::rsqrt_d1(la2);
     w.RA = la2 * w.rA;
-    w.P = w.RA + (uh2 - (u1 * w.a1 + u2 * w.a2));
     w.X = 1.0; w.Y = 0.0;
     w.sw1 = w.sw2 = w.sw3 = 0.0;
     w.x1 = w.x2 = w.sal = 0.0;
@@ -756,15 +759,16 @@ DISCO_HD void line_image_flat_f64(double q1, double q2, double h, const SegPD* _
         int k = k0;
 #pragma unroll 1
         for (; k + 1 < k1; k += 2) {
-            flat_step_d<M, GLIDE>(s[k], q1, q2, h2, u1, u2, uh2, nh, mh, w);
-            flat_step_d<M, GLIDE>(s[k + 1], q1, q2, h2, u1, u2, uh2, nh, mh, w);
+            flat_step_d<M, GLIDE>(s[k], q1, q2, h2, ah, mh, w);
+            flat_step_d<M, GLIDE>(s[k + 1], q1, q2, h2, ah, mh, w);
         }
-        if (k < k1) flat_step_d<M, GLIDE>(s[k], q1, q2, h2, u1, u2, uh2, nh, mh, w);
+        if (k < k1) flat_step_d<M, GLIDE>(s[k], q1, q2, h2, ah, mh, w);
         const int e = M::exponent_d(M::fmax_d(M::fabs_d(w.X), M::fabs_d(w.Y)));
         w.X = M::ldexp_d(w.X, -e);
         w.Y = M::ldexp_d(w.Y, -e);
     }
-    fo = 2.0 * atan2_own_d<M>(w.Y, w.X) * kInv4Pi;
+    const double om = 2.0 * atan2_own_d<M>(w.Y, w.X) * kInv4Pi;
+    fo = h > 0.0 ? -om : om;
     double x1 = w.x1, x2 = w.x2, x3 = mh * w.sal;
     if (GLIDE) { x1 *= mh; x2 *= mh; x3 *= mh; }
     o1 = kCJ * w.sw1 - kC3 * x1;

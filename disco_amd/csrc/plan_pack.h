@@ -386,7 +386,9 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, d
                 g.b1 = (float)p1[k]; g.b2 = (float)p2[k];
                 g.t1 = (float)t1; g.t2 = (float)t2;
                 g.L = (float)len2;
-                g.n3 = (float)(p1[j] * p2[k] - p2[j] * p1[k]);
+                g.iL = (float)(1.0 / len2);
+                g.e1 = (float)e1;
+                g.e2 = (float)e2;
                 g.w1 = (float)(-fb3 * t2);
                 g.w2 = (float)(fb3 * t1);
                 g.w3 = (float)(fb1 * t2 - fb2 * t1);
@@ -400,7 +402,9 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, d
                 e.b1 = p1[k]; e.b2 = p2[k];
                 e.t1 = e1 / len2; e.t2 = e2 / len2;
                 e.L = len2;
-                e.n3 = p1[j] * p2[k] - p2[j] * p1[k];
+                e.iL = 1.0 / len2;
+                e.e1 = e1;
+                e.e2 = e2;
                 e.w1 = -b3 * e.t2;
                 e.w2 = b3 * e.t1;
                 e.w3 = b1 * e.t2 - b2 * e.t1;
