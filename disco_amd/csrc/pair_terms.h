@@ -170,20 +170,24 @@ static_assert(sizeof(SegPD) == sizeof(Seg<double>), "one FP64 segment slot per s
 enum { LOOP_3D = 0, LOOP_FLAT = 1, LOOP_GLIDE = 2, LOOP_PRISM = 3 };
 
 // ------------------------------------------------------------ atanh pieces
-// 2 atanh(z) for 0 <= z < 1.  For z < 1/4: z (2 + y p(y)), y = z^2, p the
-// degree-2 minimax fit of (2 atanh(z)/z - 2)/y on [0, 1/16] (relative error
-// 1.8e-8 in exact arithmetic, below float rounding); log form above.
+// 2 atanh(z) for 0 <= z < 1.  For z < 1/8: z (2 + y p(y)), y = z^2, p the
+// degree-1 minimax fit of (2 atanh(z)/z - 2)/y on [0, 1/64] (relative error
+// 2.1e-8 in exact arithmetic, 1.3e-7 evaluated in float, as the degree-2 fit
+// on [0, 1/16] is); above it the log form.  Every lane evaluates the series
+// and the log form runs only when a lane of the wave needs it: one
+// wave-uniform branch instead of two exec-masked halves.
 template <class M>
-DISCO_HD float two_atanh_f(float z)
+DISCO_HD float two_atanh_vote_f(float z)
 {
-    if (z < 0.25f) {
-        const float y = z * z;
-        float p = 0.314437628f;
-        p = p * y + 0.39893204f;
-        p = p * y + 0.666677833f;
-        return z * (y * p + 2.0f);
+    const float y = z * z;
+    const float p = 0.40630186f * y + 0.66663605f;
+    float r = z * (y * p + 2.0f);
+    const bool big = !(z < 0.125f);
+    if (__builtin_expect(M::any(big), 0)) {
+        const float l = M::log_f((1.0f + z) * M::rcp_f(1.0f - z));
+        r = big ? l : r;
     }
-    return M::log_f((1.0f + z) * M::rcp_f(1.0f - z));
+    return r;
 }
 
 // ------------------------------------------------------------ FP64 elementary functions
@@ -350,7 +354,7 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
                 kn = mixed ? knm : kn;
             }
             const float rr = M::rcp_f(RS * kd);
-            const float J0 = two_atanh_f<M>(g.L * kd * rr);
+            const float J0 = two_atanh_vote_f<M>(g.L * kd * rr);
             const float K0 = kn * RS * rr;
             const float K1 = rA - rB;
             const float dw = ax * g.wkx + ay * g.wky + az * g.wkz;   // cK (d.w) = cK (a.w)
@@ -429,7 +433,7 @@ DISCO_HD void flat_step(const SegP& g, float q1, float q2, float h2, float ah, f
         kn = mixed ? knm : kn;
     }
     const float Lrr = g.L * M::rcp_f(RS * kd);
-    const float J0 = two_atanh_f<M>(kd * Lrr);
+    const float J0 = two_atanh_vote_f<M>(kd * Lrr);
     const float K0 = kn * (RS * Lrr);
     const float K1 = w.rA - rB;
     // d.w = a.w with a = (a1, a2, -h); LOOP_GLIDE: a.w = -h w3, the factor -h
