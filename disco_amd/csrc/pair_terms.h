@@ -146,7 +146,8 @@ struct alignas(64) SegP {
     float iL;           // 1 / L
     float w1, w2, w3;   // b x t in the frame
     float e1, e2;       // B - A
-    float pad[5];
+    float wt1, wt2;     // w3 t (the glide form's be t = (K1 - sA K0) w3 t)
+    float pad[3];
 };
 static_assert(sizeof(SegP) == sizeof(Seg<float>), "one FP32 segment slot per segment");
 
@@ -159,7 +160,8 @@ struct alignas(128) SegPD {
     double iL;          // 1 / L
     double w1, w2, w3;  // b x t in the frame
     double e1, e2;      // B - A
-    double pad[5];
+    double wt1, wt2;    // w3 t
+    double pad[3];
 };
 static_assert(sizeof(SegPD) == sizeof(Seg<double>), "one FP64 segment slot per segment");
 
@@ -394,7 +396,7 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
 // For h = 0 this gives 0, the field outside the cut; a lane on the cut itself
 // is in the near slab, which the FP64 pass evaluates.
 struct FlatWalk {
-    float a1, a2, RA, rA;                    // A - p in the plane, |A-p|, 1/|A-p|
+    float a1, a2, la2, RA, rA;               // A - p in the plane, |A-p|^2, |A-p|, 1/|A-p|
     float X, Y;                              // running solid-angle product
     float sw1, sw2, sw3;                     // sum J0 w
     float x1, x2, sal;                       // sum (al a + be t) in the plane, sum al (a3 = -h)
@@ -411,21 +413,23 @@ DISCO_HD void flat_step(const SegP& g, float q1, float q2, float h2, float ah, f
     const float rArB = w.rA * rB;
     const float RS = w.RA + RB;
 
-    const float ab = w.a1 * p1 + (w.a2 * p2 + h2);              // (A-p).(B-p)
-    const float den = (ah * RS + ab) * rArB + 1.0f;
+    // b = a + L t: a.b = |a|^2 + L sA (|a|^2 is the previous segment's |b|^2,
+    // computed from its coordinates, so nothing accumulates along the walk)
+    const float sA = w.a1 * g.t1 + w.a2 * g.t2;
+    const float den = ((ah * RS + w.la2) + g.L * sA) * rArB + 1.0f;
     const float num = (w.a1 * g.e2 - w.a2 * g.e1) * rArB;
     const float Xn = w.X * den - w.Y * num;
     w.Y = w.X * num + w.Y * den;
     w.X = Xn;
 
-    // K0 = L kn / kd, J0 = 2 atanh(L / RS): one reciprocal of RS kd serves both
-    const float sA = w.a1 * g.t1 + w.a2 * g.t2;
-    const float sB = sA + g.L;
-    const float s2 = sA + sB;
+    // K0 = L kn / kd, J0 = 2 atanh(L / RS): one reciprocal of RS kd serves
+    // both; kd = sB RA + sA RB = sA RS + L RA
+    const float s2 = 2.0f * sA + g.L;
     float kn = s2 * rArB;
-    float kd = sB * w.RA + sA * RB;
+    float kd = sA * RS + g.L * w.RA;
     const bool mixed = M::fabs_f(s2) < g.L;                     // sA < 0 < sB
     if (__builtin_expect(M::any(mixed), 0)) {                   // a branch, not selects
+        const float sB = sA + g.L;
         const float d1 = w.a1 - sA * g.t1, d2 = w.a2 - sA * g.t2;
         const float dd = d1 * d1 + (d2 * d2 + h2);
         const float knm = (sB * rB - sA * w.rA) * g.iL;
@@ -443,15 +447,22 @@ DISCO_HD void flat_step(const SegP& g, float q1, float q2, float h2, float ah, f
     else if (KIND == LOOP_PRISM) dw = w.a1 * g.w1 + w.a2 * g.w2;
     else dw = w.a1 * g.w1 + (w.a2 * g.w2 + mh * g.w3);
     const float al = dw * K0;
-    const float be = dw * (K1 - sA * K0);
     if (KIND != LOOP_GLIDE) { w.sw1 += J0 * g.w1; w.sw2 += J0 * g.w2; }
     if (KIND != LOOP_PRISM) w.sw3 += J0 * g.w3;
-    w.x1 = (w.x1 + al * w.a1) + be * g.t1;                     // two FMAs each
-    w.x2 = (w.x2 + al * w.a2) + be * g.t2;
+    if (KIND == LOOP_GLIDE) {
+        // dw = w3 is per segment: be t = (K1 - sA K0) (w3 t) from the table
+        const float c = K1 - sA * K0;
+        w.x1 = (w.x1 + al * w.a1) + c * g.wt1;
+        w.x2 = (w.x2 + al * w.a2) + c * g.wt2;
+    } else {
+        const float be = dw * (K1 - sA * K0);
+        w.x1 = (w.x1 + al * w.a1) + be * g.t1;                 // two FMAs each
+        w.x2 = (w.x2 + al * w.a2) + be * g.t2;
+    }
     w.sal += al;
 
     w.a1 = p1; w.a2 = p2;
-    w.RA = RB; w.rA = rB;
+    w.la2 = lb2; w.RA = RB; w.rA = rB;
 }
 
 // Field of one image of a flat loop, in the loop frame: fo = Omega / 4 pi
@@ -466,9 +477,9 @@ DISCO_HD void line_image_flat_f32(float q1, float q2, float h, const SegP* __res
     const SegP& last = s[nv - 1];
     w.a1 = last.b1 - q1;
     w.a2 = last.b2 - q2;
-    const float la2 = w.a1 * w.a1 + (w.a2 * w.a2 + h2);
-    w.rA = M::rsqrt_f(la2);
-    w.RA = la2 * w.rA;
+    w.la2 = w.a1 * w.a1 + (w.a2 * w.a2 + h2);
+    w.rA = M::rsqrt_f(w.la2);
+    w.RA = w.la2 * w.rA;
     w.X = 1.0f; w.Y = 0.0f;
     w.sw1 = w.sw2 = w.sw3 = 0.0f;
     w.x1 = w.x2 = w.sal = 0.0f;
@@ -652,7 +663,7 @@ DISCO_HD void line_image_f64(double qx, double qy, double qz, const SegD* __rest
 // estimates refined by Newton steps; 2 atanh from its series unless a lane of
 // the wave needs the log form.
 struct FlatWalkD {
-    double a1, a2, RA, rA;
+    double a1, a2, la2, RA, rA;
     double X, Y;
     double sw1, sw2, sw3;
     double x1, x2, sal;
@@ -699,20 +710,19 @@ DISCO_HD void flat_step_d(const SegPD& g, double q1, double q2, double h2, doubl
     const double rArB = w.rA * rB;
     const double RS = w.RA + RB;
 
-    const double ab = w.a1 * p1 + (w.a2 * p2 + h2);
-    const double den = (ah * RS + ab) * rArB + 1.0;
+    const double sA = w.a1 * g.t1 + w.a2 * g.t2;
+    const double den = ((ah * RS + w.la2) + g.L * sA) * rArB + 1.0;
     const double num = (w.a1 * g.e2 - w.a2 * g.e1) * rArB;
     const double Xn = w.X * den - w.Y * num;
     w.Y = w.X * num + w.Y * den;
     w.X = Xn;
 
-    const double sA = w.a1 * g.t1 + w.a2 * g.t2;
-    const double sB = sA + g.L;
-    const double s2 = sA + sB;
+    const double s2 = 2.0 * sA + g.L;
     double kn = s2 * rArB;
-    double kd = sB * w.RA + sA * RB;
+    double kd = sA * RS + g.L * w.RA;
     const bool mixed = M::fabs_d(s2) < g.L;                     // sA < 0 < sB
     if (__builtin_expect(M::any(mixed), 0)) {
+        const double sB = sA + g.L;
         const double d1 = w.a1 - sA * g.t1, d2 = w.a2 - sA * g.t2;
         const double dd = d1 * d1 + (d2 * d2 + h2);
         const double knm = (sB * rB - sA * w.rA) * g.iL;
@@ -725,18 +735,24 @@ DISCO_HD void flat_step_d(const SegPD& g, double q1, double q2, double h2, doubl
     const double K1 = w.rA - rB;
     const double dw = GLIDE ? g.w3 : w.a1 * g.w1 + (w.a2 * g.w2 + mh * g.w3);
     const double al = dw * K0;
-    const double be = dw * (K1 - sA * K0);
     if (!GLIDE) {
         w.sw1 += J0 * g.w1;
         w.sw2 += J0 * g.w2;
     }
     w.sw3 += J0 * g.w3;
-    w.x1 = (w.x1 + al * w.a1) + be * g.t1;
-    w.x2 = (w.x2 + al * w.a2) + be * g.t2;
+    if (GLIDE) {
+        const double c = K1 - sA * K0;
+        w.x1 = (w.x1 + al * w.a1) + c * g.wt1;
+        w.x2 = (w.x2 + al * w.a2) + c * g.wt2;
+    } else {
+        const double be = dw * (K1 - sA * K0);
+        w.x1 = (w.x1 + al * w.a1) + be * g.t1;
+        w.x2 = (w.x2 + al * w.a2) + be * g.t2;
+    }
     w.sal += al;
 
     w.a1 = p1; w.a2 = p2;
-    w.RA = RB; w.rA = rB;
+    w.la2 = lb2; w.RA = RB; w.rA = rB;
 }
 
 // Field of one image in the loop frame: fo = Omega / 4 pi (the caller adds
@@ -751,9 +767,9 @@ DISCO_HD void line_image_flat_f64(double q1, double q2, double h, const SegPD* _
     const SegPD& last = s[nv - 1];
     w.a1 = last.b1 - q1;
     w.a2 = last.b2 - q2;
-    const double la2 = w.a1 * w.a1 + (w.a2 * w.a2 + h2);
-    w.rA = M::rsqrt_d1(la2);
-    w.RA = la2 * w.rA;
+    w.la2 = w.a1 * w.a1 + (w.a2 * w.a2 + h2);
+    w.rA = M::rsqrt_d1(w.la2);
+    w.RA = w.la2 * w.rA;
     w.X = 1.0; w.Y = 0.0;
     w.sw1 = w.sw2 = w.sw3 = 0.0;
     w.x1 = w.x2 = w.sal = 0.0;

@@ -392,6 +392,8 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, d
                 g.w1 = (float)(-fb3 * t2);
                 g.w2 = (float)(fb3 * t1);
                 g.w3 = (float)(fb1 * t2 - fb2 * t1);
+                g.wt1 = (float)((fb1 * t2 - fb2 * t1) * t1);
+                g.wt2 = (float)((fb1 * t2 - fb2 * t1) * t2);
                 std::memcpy(&f, &g, sizeof(f));
             }
             if (flat64) {
@@ -408,6 +410,8 @@ inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, d
                 e.w1 = -b3 * e.t2;
                 e.w2 = b3 * e.t1;
                 e.w3 = b1 * e.t2 - b2 * e.t1;
+                e.wt1 = e.w3 * e.t1;
+                e.wt2 = e.w3 * e.t2;
                 std::memcpy(&d, &e, sizeof(d));
             }
             out.segd.push_back(d);
