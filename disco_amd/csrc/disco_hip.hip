@@ -44,6 +44,8 @@ struct DevMath {
     DISCO_HD static float atan2_f(float y, float x) { return atan2f(y, x); }
     DISCO_HD static float fabs_f(float x) { return fabsf(x); }
     DISCO_HD static float fmax_f(float a, float b) { return fmaxf(a, b); }
+    DISCO_HD static float fmin_f(float a, float b) { return fminf(a, b); }
+    DISCO_HD static float copysign_f(float a, float b) { return copysignf(a, b); }
     DISCO_HD static int exponent_f(float x) { return __builtin_amdgcn_frexp_expf(x); }
     DISCO_HD static float ldexp_f(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
     DISCO_HD static double sqrt_d(double x) { return sqrt(x); }

@@ -192,6 +192,32 @@ DISCO_HD float two_atanh_vote_f(float z)
     return r;
 }
 
+// atan2(y, x) in float for finite arguments: t = min/max of |x|, |y| in
+// [0, 1], atan(t) = t + t s q(s) with s = t^2 and q of degree 7, the
+// relative minimax fit of atan on [0, 1] (1.5e-8 in exact arithmetic, 2.4 ulp
+// evaluated in float), then the C library's quadrants and the sign of y
+// ((+-0, +-0) gives +-0).  A magnitude |(x, y)| in [1e-37, 1e37] keeps the
+// reciprocal of the larger one normal.
+template <class M>
+DISCO_HD float atan2_own_f(float y, float x)
+{
+    const float ax = M::fabs_f(x), ay = M::fabs_f(y);
+    const float t = M::fmin_f(ax, ay) * M::rcp_f(M::fmax_f(M::fmax_f(ax, ay), 1e-37f));
+    const float s = t * t;
+    float q = 0.00284985849f;
+    q = q * s - 0.0160685126f;
+    q = q * s + 0.0426913425f;
+    q = q * s - 0.0750428066f;
+    q = q * s + 0.106409281f;
+    q = q * s - 0.142036438f;
+    q = q * s + 0.199926198f;
+    q = q * s - 0.333330721f;
+    float r = (t * s) * q + t;
+    r = ay > ax ? 1.57079637f - r : r;
+    r = x < 0.0f ? 3.14159274f - r : r;
+    return M::copysign_f(r, y);
+}
+
 // ------------------------------------------------------------ FP64 elementary functions
 // log and atan2 in double from multiply-adds, one reciprocal and exponent
 // manipulation: far fewer live registers than the library routines, which set
@@ -315,10 +341,11 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
     float swx = 0.f, swy = 0.f, swz = 0.f;
     float sax = 0.f, say = 0.f, saz = 0.f;
     float stx = 0.f, sty = 0.f, stz = 0.f;
-    // blocks of 16 segments: the complex product is renormalised between
-    // blocks (each scaled factor has modulus <= 4, so 16 of them stay finite)
-    for (int k0 = 0; k0 < nv; k0 += 16) {
-        const int k1 = nv < k0 + 16 ? nv : k0 + 16;
+    // blocks of 60 segments: each scaled factor has modulus <= 4, so the
+    // product of 60 stays below 4^60 ~ 1.3e36; renormalised between blocks
+    // (atan2 takes the last block's as it is)
+    for (int k0 = 0; k0 < nv; k0 += 60) {
+        const int k1 = nv < k0 + 60 ? nv : k0 + 60;
         for (int k = k0; k < k1; k++) {
             const SegF& g = s[k];
             const float px = g.bx - qx, py = g.by - qy, pz = g.bz - qz;   // B - p
@@ -369,11 +396,13 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
             ax = px; ay = py; az = pz;
             RA = RB; rA = rB;
         }
-        const int e = M::exponent_f(M::fmax_f(M::fabs_f(X), M::fabs_f(Y)));
-        X = M::ldexp_f(X, -e);
-        Y = M::ldexp_f(Y, -e);
+        if (k1 < nv) {
+            const int e = M::exponent_f(M::fmax_f(M::fabs_f(X), M::fabs_f(Y)));
+            X = M::ldexp_f(X, -e);
+            Y = M::ldexp_f(Y, -e);
+        }
     }
-    const float fo = 2.0f * M::atan2_f(Y, X) * (float)kInv4Pi + (float)wind;
+    const float fo = 2.0f * atan2_own_f<M>(Y, X) * (float)kInv4Pi + (float)wind;
     ox += bx * fo + (swx - sax - stx);
     oy += by * fo + (swy - say - sty);
     oz += bz * fo + (swz - saz - stz);
@@ -483,21 +512,24 @@ DISCO_HD void line_image_flat_f32(float q1, float q2, float h, const SegP* __res
     w.X = 1.0f; w.Y = 0.0f;
     w.sw1 = w.sw2 = w.sw3 = 0.0f;
     w.x1 = w.x2 = w.sal = 0.0f;
-    // blocks of 16 segments, renormalised between blocks; two segments per
-    // iteration (fewer register copies of the carried A - p)
-    for (int k0 = 0; k0 < nv; k0 += 16) {
-        const int k1 = nv < k0 + 16 ? nv : k0 + 16;
+    // blocks of 60 segments (|g| <= 4: a product of 60 stays below 1.3e36),
+    // renormalised between blocks; two segments per iteration (fewer
+    // register copies of the carried A - p)
+    for (int k0 = 0; k0 < nv; k0 += 60) {
+        const int k1 = nv < k0 + 60 ? nv : k0 + 60;
         int k = k0;
         for (; k + 1 < k1; k += 2) {
             flat_step<M, KIND>(s[k], q1, q2, h2, ah, mh, w);
             flat_step<M, KIND>(s[k + 1], q1, q2, h2, ah, mh, w);
         }
         for (; k < k1; k++) flat_step<M, KIND>(s[k], q1, q2, h2, ah, mh, w);
-        const int e = M::exponent_f(M::fmax_f(M::fabs_f(w.X), M::fabs_f(w.Y)));
-        w.X = M::ldexp_f(w.X, -e);
-        w.Y = M::ldexp_f(w.Y, -e);
+        if (k1 < nv) {
+            const int e = M::exponent_f(M::fmax_f(M::fabs_f(w.X), M::fabs_f(w.Y)));
+            w.X = M::ldexp_f(w.X, -e);
+            w.Y = M::ldexp_f(w.Y, -e);
+        }
     }
-    const float om = 2.0f * M::atan2_f(w.Y, w.X) * (float)kInv4Pi;
+    const float om = 2.0f * atan2_own_f<M>(w.Y, w.X) * (float)kInv4Pi;
     fo = h > 0.0f ? -om : om;
     float x1 = w.x1, x2 = w.x2, x3 = mh * w.sal;
     if (KIND == LOOP_GLIDE) { x1 *= mh; x2 *= mh; x3 *= mh; }

@@ -1,8 +1,9 @@
-"""The FP64 elementary functions of the kernels (log for w >= 1, atan2, in
-disco_amd/csrc/pair_terms.h), compiled for the host with the same source and
-checked against the C library over wide argument ranges: at most a few units
-in the last place.  (The device build runs the same code; the FP64 parity tests
-check it there through the sums.)"""
+"""The elementary functions of the kernels (FP64 log for w >= 1 and atan2,
+FP32 atan2, in disco_amd/csrc/pair_terms.h), compiled for the host with the
+same source and checked against the C library over wide argument ranges: at
+most a few units in the last place.  (The device build runs the same code, with
+the hardware reciprocal in place of the host's division; the parity tests check
+it there through the sums.)"""
 import os
 import shutil
 import subprocess
@@ -22,6 +23,11 @@ struct double4 { double x, y, z, w; };
 #include "pair_terms.h"
 
 struct HostMath {
+    static float rcp_f(float x) { return 1.0f / x; }
+    static float fabs_f(float x) { return std::fabs(x); }
+    static float fmax_f(float a, float b) { return std::fmax(a, b); }
+    static float fmin_f(float a, float b) { return std::fmin(a, b); }
+    static float copysign_f(float a, float b) { return std::copysign(a, b); }
     static double rcp_d(double x) { return 1.0 / x; }
     static double fabs_d(double x) { return std::fabs(x); }
     static bool signbit_d(double x) { return std::signbit(x); }
@@ -57,7 +63,27 @@ int main()
             const double a = disco::atan2_own_d<HostMath>(y, x), b = std::atan2(y, x);
             if (std::signbit(a) != std::signbit(b) || ulps(a, b) > 4) bad_signs++;
         }
-    std::printf("%.3f %.3f %ld\n", worst_log, worst_atan, bad_signs);
+    // FP32 atan2 against the double library value rounded to float's ulp, over
+    // the magnitudes the walks hand it (1e-18 .. 1e36) and every angle
+    double worst_f = 0;
+    std::uniform_real_distribution<double> A(-3.14159265358979, 3.14159265358979);
+    for (int i = 0; i < 4000000; i++) {
+        const double m = std::pow(10.0, -18.0 + 54.0 * (0.5 + 0.5 * U(r)));
+        const double an = (i & 1) ? A(r) : std::ldexp(U(r), -(int)(24.0 * std::fabs(U(r))));
+        const float x = (float)(m * std::cos(an)), y = (float)(m * std::sin(an));
+        const double want = std::atan2((double)y, (double)x);
+        if (want == 0.0) continue;
+        const double u = std::ldexp(1.0, std::ilogb(want) - 23);
+        worst_f = std::fmax(worst_f, std::fabs(disco::atan2_own_f<HostMath>(y, x) - want) / u);
+    }
+    for (float y : {0.0f, -0.0f, 1.0f, -1.0f, 3e-18f})
+        for (float x : {0.0f, 1.0f, -1.0f, 2e30f, -5e-10f}) {
+            const float a = disco::atan2_own_f<HostMath>(y, x);
+            const double b = std::atan2((double)y, (double)x);
+            if (x == 0.0f && y == 0.0f) { if (a != 0.0f || std::signbit(a) != std::signbit(y)) bad_signs++; continue; }
+            if (std::signbit(a) != std::signbit(b) || std::fabs(a - b) > 4e-7 * std::fmax(std::fabs(b), 1e-30)) bad_signs++;
+        }
+    std::printf("%.3f %.3f %ld %.3f\n", worst_log, worst_atan, bad_signs, worst_f);
     return 0;
 }
 '''
@@ -74,6 +100,8 @@ def test_fp64_log_and_atan2_match_the_c_library(tmp_path):
                     str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
     worst_log, worst_atan, bad = float(out[0]), float(out[1]), int(out[2])
+    worst_f = float(out[3])
     assert worst_log <= 4.0, worst_log
     assert worst_atan <= 4.0, worst_atan
+    assert worst_f <= 3.0, worst_f
     assert bad == 0
