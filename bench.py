@@ -35,8 +35,8 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak (packed FP
 FP32_UNPACKED_PEAK_TFLOPS = 78.6  # the same without packed FP32 (one FMA per lane per clock)
 # HBM bytes per atom of the far pass, measured with rocprofv3 PMC counters
 # (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, each scaled by a plain copy of
-# known size, `--workload copy`): 193.9e6 B per dispatch over 4,000,000 atoms.
-FAR_PASS_HBM_BYTES_PER_ATOM = 48.49
+# known size, `--workload copy`): 194.0e6 B per dispatch over 4,000,000 atoms.
+FAR_PASS_HBM_BYTES_PER_ATOM = 48.50
 # cubature form of the sample (reported beside the plain sum): exact within 5 cut
 # radii of a loop image, the 9-point cubature beyond; +-1 images far from every
 # atom on an L/32 grid (interpolation error bound 1e-7 A each), the far shell on
