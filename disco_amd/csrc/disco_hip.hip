@@ -1011,15 +1011,22 @@ int host_sum(disco_ctx* ctx, disco_plan* p, const std::vector<double>& work, int
     double* dx = ctx->d_atoms;
     double *dy = dx + n, *dz = dy + n, *dux = dz + n, *duy = dux + n, *duz = duy + n;
     const size_t bytes = sizeof(double) * (size_t)n;
-    DISCO_HIP_TRY(ctx, hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, ctx->stream));
-    DISCO_HIP_TRY(ctx, hipMemcpyAsync(dy, y, bytes, hipMemcpyHostToDevice, ctx->stream));
-    DISCO_HIP_TRY(ctx, hipMemcpyAsync(dz, z, bytes, hipMemcpyHostToDevice, ctx->stream));
-    DISCO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    // u is accumulated on the device: the caller's u goes up with the atoms,
+    // every launch adds to it, and the result comes straight back into u
+    const bool any_work = n > 0 && p->nloops > 0;
+    if (any_work) {
+        const double* src[6] = {x, y, z, ux, uy, uz};
+        double* dst[6] = {dx, dy, dz, dux, duy, duz};
+        for (int a = 0; a < 6; a++)
+            DISCO_HIP_TRY(ctx, hipMemcpyAsync(dst[a], src[a], bytes, hipMemcpyHostToDevice,
+                                              ctx->stream));
+        DISCO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
     const double t1 = now_s();
     // with per-loop times asked for, one launch per loop timed by its own events
     // (the reference's kroupaGPU launches and times each loop:
     // Code/backend_cuda.cu:643-700); u then sums loop by loop, as kroupaCPU does
-    const bool per_loop = t && t->loop_sec && p->nloops > 0 && n > 0;
+    const bool per_loop = t && t->loop_sec && any_work;
     std::vector<hipEvent_t> lev;
     if (per_loop) {
         lev.assign((size_t)p->nloops + 1, nullptr);
@@ -1037,13 +1044,12 @@ int host_sum(disco_ctx* ctx, disco_plan* p, const std::vector<double>& work, int
     if (per_loop) {
         if (hipEventRecord(lev[0], ctx->stream) != hipSuccess) rc = fail(ctx, DISCO_ERR_HIP, "hipEventRecord");
         for (int j = 0; j < p->nloops && rc == DISCO_OK; j++) {
-            rc = launch(p, n, dx, dy, dz, dux, duy, duz, j > 0 ? 1 : 0, ctx->stream, nullptr, 0,
-                        j, j + 1);
+            rc = launch(p, n, dx, dy, dz, dux, duy, duz, 1, ctx->stream, nullptr, 0, j, j + 1);
             if (rc == DISCO_OK && hipEventRecord(lev[j + 1], ctx->stream) != hipSuccess)
                 rc = fail(ctx, DISCO_ERR_HIP, "hipEventRecord");
         }
-    } else {
-        rc = launch(p, n, dx, dy, dz, dux, duy, duz, 0, ctx->stream);
+    } else if (any_work) {
+        rc = launch(p, n, dx, dy, dz, dux, duy, duz, 1, ctx->stream);
     }
     if (rc != DISCO_OK) { destroy(); return rc; }
     if (hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess ||
@@ -1064,17 +1070,11 @@ int host_sum(disco_ctx* ctx, disco_plan* p, const std::vector<double>& work, int
     }
     destroy();
     DISCO_HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    // download into host staging, then accumulate into the caller's u
-    std::vector<double> h(3 * (size_t)n);
-    if (n > 0 && p->nloops > 0) {
-        DISCO_HIP_TRY(ctx, hipMemcpyAsync(h.data(), dux, 3 * bytes, hipMemcpyDeviceToHost,
-                                          ctx->stream));
+    if (any_work) {
+        DISCO_HIP_TRY(ctx, hipMemcpyAsync(ux, dux, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        DISCO_HIP_TRY(ctx, hipMemcpyAsync(uy, duy, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        DISCO_HIP_TRY(ctx, hipMemcpyAsync(uz, duz, bytes, hipMemcpyDeviceToHost, ctx->stream));
         DISCO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        for (int a = 0; a < n; a++) {
-            ux[a] += h[a];
-            uy[a] += h[(size_t)n + a];
-            uz[a] += h[2 * (size_t)n + a];
-        }
     }
     const double t3 = now_s();
     if (t) {

@@ -110,8 +110,9 @@ const char *disco_hip_last_error(const disco_ctx *ctx);
 
 /* ---------------------------------------------------------------- host buffers
  * One call = the reference's kroupaGPU (Code/backend_cuda.cu:568-722) for the
- * line (resp. loopon) loops: upload the atoms, sum every loop in one launch,
- * download.  u is accumulated (u += sum), as kroupaCPU does
+ * line (resp. loopon) loops: upload the atoms and u, sum every loop in one
+ * launch (one launch per loop, each timed, when t->loop_sec is given), add to u
+ * on the device, download u.  u is accumulated (u += sum), as kroupaCPU does
  * (Code/backend_cpu.cpp:57-59); with a zeroed u this equals the reference
  * GPU's overwrite (Code/backend_cuda.cu:710-712).  t may be NULL. */
 int disco_hip_line_sum(disco_ctx *ctx, int n, const double *x, const double *y,
