@@ -994,7 +994,7 @@ int make_loopon_plan(disco_ctx* ctx, const disco_loopon_loop* loops, int nloops,
 
 // Host-buffer sum: upload atoms (and u when accumulating into non-zero u is
 // needed: u is always accumulated on the host side), run, download.
-int host_sum(disco_ctx* ctx, disco_plan* p, const std::vector<double>& work, int n,
+int host_sum(disco_ctx* ctx, disco_plan* p, int n,
              const double* x, const double* y, const double* z, double* ux, double* uy,
              double* uz, disco_timing* t, int nloops_in)
 {
@@ -1389,10 +1389,9 @@ int disco_hip_line_sum(disco_ctx* ctx, int n, const double* x, const double* y, 
     if (n > 0 && (!x || !y || !z || !ux || !uy || !uz)) return fail(ctx, DISCO_ERR_ARG, "null atom array");
     if (nloops > 0 && !loops) return fail(ctx, DISCO_ERR_ARG, "null loop array");
     disco_plan* p = nullptr;
-    std::vector<double> work;
-    rc = make_line_plan(ctx, loops, nloops, Lx, Ly, Lz, img, precision, &p, &work);
+    rc = make_line_plan(ctx, loops, nloops, Lx, Ly, Lz, img, precision, &p, nullptr);
     if (rc != DISCO_OK) return rc;
-    rc = host_sum(ctx, p, work, n, x, y, z, ux, uy, uz, t, nloops);
+    rc = host_sum(ctx, p, n, x, y, z, ux, uy, uz, t, nloops);
     free_plan(p);
     return rc;
 }
@@ -1407,10 +1406,9 @@ int disco_hip_loopon_sum(disco_ctx* ctx, int n, const double* x, const double* y
     if (n > 0 && (!x || !y || !z || !ux || !uy || !uz)) return fail(ctx, DISCO_ERR_ARG, "null atom array");
     if (nloops > 0 && !loops) return fail(ctx, DISCO_ERR_ARG, "null loop array");
     disco_plan* p = nullptr;
-    std::vector<double> work;
-    rc = make_loopon_plan(ctx, loops, nloops, Lx, Ly, Lz, img, precision, &p, &work);
+    rc = make_loopon_plan(ctx, loops, nloops, Lx, Ly, Lz, img, precision, &p, nullptr);
     if (rc != DISCO_OK) return rc;
-    rc = host_sum(ctx, p, work, n, x, y, z, ux, uy, uz, t, nloops);
+    rc = host_sum(ctx, p, n, x, y, z, ux, uy, uz, t, nloops);
     free_plan(p);
     return rc;
 }
