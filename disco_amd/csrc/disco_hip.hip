@@ -427,7 +427,7 @@ __device__ __forceinline__ void atom_loopon(double px, double py, double pz,
 // loop-frame walk, two segments per iteration, fits 168 registers (at 4 waves
 // and 128 registers the walk needs one segment per iteration and register
 // copies of the carried terms: 2 % slower, the same bits); the FP32 ones use
-// about 80 and are unaffected.
+// about 70 and are unaffected.
 template <int PASS, int KIND, bool CUBATURE = false>
 __global__ __launch_bounds__(kBlock, 3) void line_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
