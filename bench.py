@@ -11,7 +11,8 @@ geometry is uploaded once (a plan).
 Prints one JSON line: evals/s (atom x segment x image evaluations per second),
 ms per 4M-atom sample, the error of the timed output against the FP64 sum, the
 roofline of the dominant kernel (HIP events around it, live) and the CPU
-baseline (the FP64 oracle on the host cores, on a bounded atom sample).
+baseline (the reference's own kroupaCPU, compiled from its sources, and the FP64
+oracle, on the host cores, on bounded atom samples).
 """
 from __future__ import annotations
 
@@ -77,24 +78,50 @@ def host_cpu():
             "host_logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
 
-def cpu_baseline(sample, stride, runs=3):
-    """FP64 oracle (the reference's loop structure, OpenMP) on every `stride`-th
-    atom: the median of `runs` timed runs."""
-    import oracle
-    idx = np.arange(0, sample.n, stride)
-    x, y, z = sample.x[idx], sample.y[idx], sample.z[idx]
+def _time_runs(fn, runs):
     times = []
     for _ in range(runs):
         t0 = time.perf_counter()
-        oracle.line_sum(x, y, z, sample.loops, sample.box, sample.img)
+        fn()
         times.append(time.perf_counter() - t0)
-    dt = float(np.median(times))
-    evals = idx.size * sample.segments() * (2 * sample.img + 1) ** 3
-    out = {"value": evals / dt, "unit": "evals/s", "cores": oracle.threads(), "kind": "port",
-           "sample": f"every {stride}th atom ({idx.size} atoms) of the same workload, all "
-                     f"{len(sample.loops)} loops, images +-{sample.img}; median of {runs} "
-                     f"runs, {dt:.1f} s each",
-           "seconds": dt, "runs": times, "threads": oracle.threads()}
+    return float(np.median(times)), times
+
+
+def cpu_baseline(sample, stride, runs=3, ref_stride=64):
+    """The CPU path timed on the host cores, each figure the median of `runs`
+    runs on a bounded atom sample of the same workload:
+      * the reference's own kroupaCPU (Code/backend_cpu.cpp:28-99, compiled from
+        its sources with its OpenMP flags), when that build is present: every
+        `ref_stride`-th atom, all loops, the +-1 images it sums (its image range
+        is fixed);
+      * the FP64 oracle (the same loop structure, generalised to +-img) on every
+        `stride`-th atom at the workload's +-img ("port").
+    `value` is the reference's rate when it was timed, else the port's."""
+    import oracle
+    segs = sample.segments()
+    idx = np.arange(0, sample.n, stride)
+    x, y, z = sample.x[idx], sample.y[idx], sample.z[idx]
+    dt, times = _time_runs(lambda: oracle.line_sum(x, y, z, sample.loops, sample.box,
+                                                   sample.img), runs)
+    port = {"value": idx.size * segs * (2 * sample.img + 1) ** 3 / dt, "unit": "evals/s",
+            "cores": oracle.threads(), "kind": "port",
+            "sample": f"every {stride}th atom ({idx.size} atoms) of the same workload, all "
+                      f"{len(sample.loops)} loops, images +-{sample.img}; median of {runs} "
+                      f"runs, {dt:.1f} s each", "seconds": dt, "runs": times}
+    if oracle.ref_available():
+        ridx = np.arange(0, sample.n, ref_stride)
+        rx, ry, rz = sample.x[ridx], sample.y[ridx], sample.z[ridx]
+        rdt, rtimes = _time_runs(lambda: oracle.ref_kroupa_cpu_line(rx, ry, rz, sample.loops,
+                                                                     sample.box), runs)
+        out = {"value": ridx.size * segs * 27 / rdt, "unit": "evals/s",
+               "cores": oracle.threads(), "kind": "reference",
+               "sample": f"the reference's kroupaCPU on every {ref_stride}th atom "
+                         f"({ridx.size} atoms) of the same workload, all {len(sample.loops)} "
+                         f"loops, images +-1 (its fixed range); median of {runs} runs, "
+                         f"{rdt:.1f} s each", "seconds": rdt, "runs": rtimes, "port": port}
+    else:
+        out = dict(port)
+    out["threads"] = oracle.threads()
     out.update(host_cpu())
     return out
 
@@ -288,8 +315,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(s, args.cpu_stride, args.cpu_runs)
-        print(f"[bench] cpu baseline: {cpu['value']:.3e} evals/s on {cpu['cores']} threads",
-              flush=True)
+        extra = (f" (port at +-{s.img}: {cpu['port']['value']:.3e})" if "port" in cpu else "")
+        print(f"[bench] cpu baseline ({cpu['kind']}): {cpu['value']:.3e} evals/s on "
+              f"{cpu['cores']} threads{extra}", flush=True)
     out = {"metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
            "ms_per_sample": ms_step, "max_abs_err_A": err, "higher_is_better": True,
