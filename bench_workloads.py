@@ -41,6 +41,23 @@ def _cpu_line(sample, stride, img=None):
     return ev / dt, oracle.threads(), f"every {stride}th atom ({idx.size}), images +-{img}, {dt:.1f} s"
 
 
+def _cpu_reference(kind, sample, stride):
+    """The reference's own kroupaCPU (Code/backend_cpu.cpp:28-99, compiled from
+    its sources) on every `stride`-th atom at its +-1 images, or None when that
+    build is absent: (evals/s, threads, description)."""
+    import oracle
+    if not oracle.ref_available():
+        return None
+    idx = np.arange(0, sample.n, stride)
+    fn = oracle.ref_kroupa_cpu_line if kind == "line" else oracle.ref_kroupa_cpu_loopon
+    t0 = time.perf_counter()
+    fn(sample.x[idx], sample.y[idx], sample.z[idx], sample.loops, sample.box)
+    dt = time.perf_counter() - t0
+    ev = idx.size * sample.segments() * 27
+    return ev / dt, oracle.threads(), (f"the reference's kroupaCPU on every {stride}th atom "
+                                       f"({idx.size}), images +-1, {dt:.1f} s")
+
+
 def run_c3(args):
     import torch
     import oracle
@@ -81,6 +98,10 @@ def run_c3(args):
         cpu = {"value": sub.size * s.segments() * 27 / dt, "unit": "evals/s",
                "cores": oracle.threads(), "kind": "port",
                "sample": f"every 128th atom ({sub.size}), all 20 loops, +-1; {dt:.1f} s"}
+        ref = _cpu_reference("loopon", s, 128)
+        if ref is not None:
+            cpu = {"value": ref[0], "unit": "evals/s", "cores": ref[1], "kind": "reference",
+                   "sample": ref[2] + ", all 20 loops", "port": cpu}
     k = float(np.mean(ms))
     ach = evals * FLOP_LOOPON / (k * 1e-3) / 1e12
     _emit({"metric": "atom·loopon·image evals/s (loopon method)", "value": evals / (k * 1e-3),
@@ -167,6 +188,10 @@ def run_c4(args):
         v, c, desc = _cpu_line(s0, 64)
         cpu = {"value": v, "unit": "evals/s", "cores": c, "kind": "port",
                "sample": "sample 0 of the batch, " + desc}
+        ref = _cpu_reference("line", s0, 64)
+        if ref is not None:
+            cpu = {"value": ref[0], "unit": "evals/s", "cores": ref[1], "kind": "reference",
+                   "sample": "sample 0 of the batch, " + ref[2], "port": cpu}
     ach = evals * FLOP_LINE / (ms * 1e-3) / 1e12
     _emit({"metric": "atom·segment·image evals/s (batched samples)", "value": evals / (ms * 1e-3),
            "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
