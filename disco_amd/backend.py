@@ -5,8 +5,8 @@ Mirrors ``Code/backend.hpp``:
 * ``LoopData``       -- Code/backend.hpp:20-26 (loopons SoA or polyline + apex)
 * ``BackendTiming``  -- Code/backend.hpp:36-39
 * ``GpuOptions``     -- Code/backend.hpp:53-56 (``fp32`` selects DISCO_PREC_FP32C;
-                        ``chunk_pairs`` is accepted and ignored: one launch covers
-                        all loops, and no watchdog needs chunking here)
+                        ``chunk_pairs`` is accepted and ignored: each loop's pass
+                        is one launch over all atoms, and no watchdog needs chunking)
 * ``gpu_available``, ``print_gpu_info``, ``gpu_line_fp64``, ``kroupa_gpu``
                      -- Code/backend.hpp:57-63 / Code/backend_cuda.cu:525-722
 
@@ -372,7 +372,8 @@ def line_cubature(loops) -> np.ndarray:
 
 
 def line_sum(ctx, x, y, z, loops, box, img=1, prec=PREC_FP64, u=None, timing=None):
-    """Host-buffer line sum: u (+)= sum over loops x images (one launch)."""
+    """Host-buffer line sum: u (+)= sum over loops x images (one launch; one per
+    loop, each timed, when `timing` is given)."""
     n = len(x)
     u = (np.zeros(n), np.zeros(n), np.zeros(n)) if u is None else u
     return _sum_host(ctx, "line", x, y, z, u, loops, box, img, prec, timing)

@@ -18,8 +18,8 @@
 // and print-and-exit on any device error (Code/backend_cuda.cu:45-53).  The
 // displacement arrays end holding the sum (the caller zeroes them,
 // Code/main.cpp:714).  GpuOptions::fp32 selects DISCO_PREC_FP32C;
-// GpuOptions::chunkPairs is not needed (one launch per method, no watchdog)
-// and is reported when set.
+// GpuOptions::chunkPairs is not needed (each loop is one launch over all
+// atoms, and there is no watchdog) and is reported when set.
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
@@ -76,7 +76,7 @@ void kroupaGPU(const AtomArrays& atoms, const vector<LoopData>& loops, double dx
          << (opt.fp32 ? "single (fp32 pair terms, fp64 image sums, fp64 near the cut)" : "double")
          << "\n";
     if (opt.chunkPairs > 0.)
-        cout << "Chunking : not needed on this backend (one launch per method); ignored\n";
+        cout << "Chunking : not needed on this backend (one launch per loop); ignored\n";
 
     disco_ctx* ctx = nullptr;
     DISCO_CHECK(nullptr, disco_hip_open(0, &ctx));
