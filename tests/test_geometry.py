@@ -85,7 +85,8 @@ def test_lattice_bit_identical_to_compiled_reference_at_workload_sizes(lat, a0, 
 
 
 def test_c1_loop_centres_match_the_measured_snap():
-    """Centres of the C1 loops (SURVEY probe: d = 2.019002 A)."""
+    """Centres of the C1 loops after the reference's mid-plane snap, as the
+    compiled reference places them (snap distance d = 2.019002 A)."""
     s = G.config_c1()
     c = np.array([[l.cx, l.cy, l.cz] for l in s.loops])
     assert np.abs(c - np.array([[-27.839175, -29.266825, -28.553],
