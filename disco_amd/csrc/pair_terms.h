@@ -422,8 +422,8 @@ DISCO_HD void line_image_f32(float qx, float qy, float qz, const SegF* __restric
 // (a x b).n = a x (B - A) from the per-segment edge B - A: free of the
 // cancellation of a1 b2 - a2 b1 (terms |q|^2 for a result |q| L) for a lane
 // far from the loop.
-// For h = 0 this gives 0, the field outside the cut; a lane on the cut itself
-// is in the near slab, which the FP64 pass evaluates.
+// A lane at h = 0 outside the contour gets 0, as it should; one on the cut
+// itself (h = 0 inside) is in the near slab, which the FP64 pass evaluates.
 struct FlatWalk {
     float a1, a2, la2, RA, rA;               // A - p in the plane, |A-p|^2, |A-p|, 1/|A-p|
     float X, Y;                              // running solid-angle product
