@@ -38,11 +38,12 @@ FP32_UNPACKED_PEAK_TFLOPS = 78.6  # the same without packed FP32 (one FMA per la
 # (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, each scaled by a plain copy of
 # known size, `--workload copy`): 194.0e6 B per dispatch over 4,000,000 atoms.
 FAR_PASS_HBM_BYTES_PER_ATOM = 48.50
-# cubature form of the sample (reported beside the plain sum): exact within 5 cut
-# radii of a loop image, the 9-point cubature beyond; +-1 images far from every
-# atom on an L/32 grid (interpolation error bound 1e-7 A each), the far shell on
-# an L/16 grid, both interpolated by 6-node stencils
-K_EXACT, NEAR_TOL = 5.0, 1e-7
+# cubature form of the sample (reported beside the plain sum): exact within 3 cut
+# radii of a loop image, the two-ring cubature from 3 to 5 radii (loops it
+# represents; the others stay exact there), the 9-point cubature beyond; +-1
+# images far from every atom on an L/32 grid (interpolation error bound 1e-7 A
+# each), the far shell on an L/16 grid, both interpolated by 6-node stencils
+K_EXACT, K_BAND, NEAR_TOL = 5.0, 3.0, 1e-7
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on this workload: "
                   "profiles/line_kernel_far__fcc4m_50loops_img2__pmc_traffic.md")
 
@@ -247,13 +248,15 @@ def main():
         print(f"[bench] max|du| vs FP64 over all {s.n} atoms: {err:.3e} A; FP64 mode: "
               f"{ms64:.1f} ms ({fp64['evals_per_s']:.3e} evals/s)", flush=True)
 
-    # the same sample in cubature form (exact within K_EXACT loop radii of a loop
-    # image, the 9-point cubature beyond, far images on grids): ms per sample and
-    # its error against the FP64 plain sum over every atom
+    # the same sample in cubature form (exact within K_BAND loop radii of a loop
+    # image, the two-ring cubature to K_EXACT, the 9-point cubature beyond, far
+    # images on grids): ms per sample and its error against the FP64 plain sum
+    # over every atom
     cub = None
     if not args.no_cubature:
         from disco_amd import cubature as C
-        cs = C.CubatureSum(ctx, x, y, z, s.loops, s.box, s.img, K_EXACT, near_tol=NEAR_TOL)
+        cs = C.CubatureSum(ctx, x, y, z, s.loops, s.box, s.img, K_EXACT, near_tol=NEAR_TOL,
+                           k_band=K_BAND)
         w = [torch.zeros_like(x) for _ in range(3)]
         cs.run(x, y, z, *w)
         torch.cuda.synchronize()
@@ -266,11 +269,14 @@ def main():
         cms = e0.elapsed_time(e1) / args.steps
         cerr = max(float((a - b).abs().max()) for a, b in zip(w, v)) if v is not None else None
         cub = {"ms_per_sample": cms, "max_abs_err_A": cerr, "err_atoms": s.n if v is not None else None,
-               "speedup_vs_plain": ms_step / cms, "k_exact": K_EXACT, "near_tol_A": NEAR_TOL,
-               "rule_loops": cs.eligible, "images_on_grid": cs.skipped,
-               "form": "exact line integral within k_exact R of each loop image (FP32, FP64 "
-                       "near slab), 9-point cubature of the cut beyond; +-1 images far from "
-                       "every atom on an L/32 grid, the far shell on an L/16 grid (6-node stencils)"}
+               "speedup_vs_plain": ms_step / cms, "k_exact": K_EXACT, "k_band": K_BAND,
+               "near_tol_A": NEAR_TOL, "rule_loops": cs.eligible, "band_loops": cs.band,
+               "images_on_grid": cs.skipped,
+               "form": "exact line integral within k_band R of each loop image (FP32, FP64 "
+                       "near slab; to k_exact R for loops the two-ring cubature does not "
+                       "represent), the two-ring cubature (25 points) to k_exact R, the 9-point "
+                       "cubature of the cut beyond; +-1 images far from every atom on an L/32 "
+                       "grid, the far shell on an L/16 grid (6-node stencils)"}
         cs.free()
         del w
         print(f"[bench] cubature form: {cms:.1f} ms per sample ({ms_step / cms:.1f}x the plain "

@@ -9,8 +9,9 @@ Each prints one JSON line in bench.py's format.  The default bench line (the
         segments: circles, rectangles, parallelograms), images +-1, one
         persistent launch per pass replayed as a HIP graph, autotuned shape.
 * c5 -- images +-4: FCC 100^3 (4M atoms), 200 loops as the default sample,
-        in cubature form (exact near each loop image, a 7-point cubature of the
-        cut far from it, far images on grids), against the FP64 plain sum.
+        in cubature form (exact near each loop image, a two-ring and then a
+        9-point cubature of the cut farther out, far images on grids), against
+        the FP64 plain sum.
 """
 from __future__ import annotations
 
@@ -211,8 +212,10 @@ def run_c4(args):
 
 def run_c5(args):
     """The 200-loop sample at images +-4 (729 per loop) in cubature form: exact
-    line integral within 5 cut radii of each loop image (FP32, FP64 near slab),
-    the 9-point cubature of the cut beyond, the +-1 images far from every atom
+    line integral within 3 cut radii of each loop image (FP32, FP64 near slab;
+    5 radii for the hexagons, which the two-ring cubature does not represent),
+    the two-ring cubature from 3 to 5 radii of the circles, the 9-point cubature
+    of the cut beyond, the +-1 images far from every atom
     on an L/32 grid and the far shell (images 2..4) on an L/16 grid (6-node
     interpolation stencils).
     Error against the FP64 plain sum on every 8th atom (every atom with
@@ -222,7 +225,7 @@ def run_c5(args):
     from disco_amd import backend as B
     from disco_amd import cubature as C
     from disco_amd import geometry as G
-    img, k_exact, near_tol = 4, 5.0, 1e-7
+    img, k_exact, k_band, near_tol = 4, 5.0, 3.0, 1e-7
     t0 = time.time()
     c = G.config_c2(seed=20245, nloops=200)
     print(f"[c5] workload {c.n} atoms, {len(c.loops)} loops, {c.segments()} segments "
@@ -231,7 +234,8 @@ def run_c5(args):
     x, y, z = (torch.from_numpy(a).to(dev) for a in (c.x, c.y, c.z))
     ctx = B.Context(0)
     t0 = time.perf_counter()
-    cs = C.CubatureSum(ctx, x, y, z, c.loops, c.box, img, k_exact, near_tol=near_tol)
+    cs = C.CubatureSum(ctx, x, y, z, c.loops, c.box, img, k_exact, near_tol=near_tol,
+                       k_band=k_band)
     t_setup = time.perf_counter() - t0
     u = [torch.zeros_like(x) for _ in range(3)]
     for _ in range(max(1, args.warmup)):
@@ -303,8 +307,9 @@ def run_c5(args):
                     "cubature on grids",
            "data": "synthetic",
            "config": {"workload": "FCC 100^3 (4M atoms), 200 loops as the 4M-atom sample, "
-                                  "images +-4", "k_exact": k_exact, "near_tol_A": near_tol,
-                      "rule_loops": cs.eligible, "images_on_grid": cs.skipped,
+                                  "images +-4", "k_exact": k_exact, "k_band": k_band,
+                      "near_tol_A": near_tol, "rule_loops": cs.eligible, "band_loops": cs.band,
+                      "images_on_grid": cs.skipped,
                       "setup_s": t_setup, "plain_gpu_ms": t_plain,
                       "speedup_vs_plain": t_plain / t_step, "plain_evals": plain_evals,
                       "plain_evals_per_s_equivalent": plain_evals / (t_step * 1e-3)},

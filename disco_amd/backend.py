@@ -158,6 +158,8 @@ def lib():
         L.disco_hip_plan_run_noalloc.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]
         L.disco_hip_plan_set_cubature.argtypes = [_vp, _d, _d, ctypes.POINTER(_i)]
         L.disco_hip_line_cubature.argtypes = [ctypes.POINTER(_LineLoop), _i, _dp]
+        L.disco_hip_plan_set_band.argtypes = [_vp, _d, ctypes.POINTER(_i)]
+        L.disco_hip_line_band.argtypes = [ctypes.POINTER(_LineLoop), _i, _dp]
         L.disco_hip_plan_set_image_skip.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint), _i]
         L.disco_hip_plan_profile.argtypes = [_vp, _i]
         L.disco_hip_plan_kernel_ms.argtypes = [_vp, _dp, _i]
@@ -363,10 +365,21 @@ def _sum_host(ctx, kind, x, y, z, u, loops, box, img, prec, timing):
 
 def line_cubature(loops) -> np.ndarray:
     """Per loop: (eligible, centre weight, ring weight, ring radius, cut radius)
-    of the 7-point cubature the cubature form uses (host only)."""
+    of the 9-point cubature the cubature form uses (host only)."""
     arr, keep = line_structs(loops)
     out = np.zeros((max(1, len(loops)), 5))
     _check(lib().disco_hip_line_cubature(arr, len(loops), out.ctypes.data_as(_dp)))
+    del keep
+    return out[:len(loops)]
+
+
+def line_band(loops) -> np.ndarray:
+    """Per loop: (eligible, centre weight, per-point weight of ring 1, of ring 2,
+    radius of ring 1, of ring 2) of the two-ring cubature of the band form
+    (host only)."""
+    arr, keep = line_structs(loops)
+    out = np.zeros((max(1, len(loops)), 6))
+    _check(lib().disco_hip_line_band(arr, len(loops), out.ctypes.data_as(_dp)))
     del keep
     return out[:len(loops)]
 
@@ -408,13 +421,22 @@ class Plan:
 
     def set_cubature(self, k_exact: float, k_rule: float | None = None) -> int:
         """Cubature form of the far pass (FP32C line plans): exact within k_exact
-        loop radii of a loop image, the 7-point cubature beyond k_rule (default
+        loop radii of a loop image, the 9-point cubature beyond k_rule (default
         k_exact: a hard switch), a smooth blend between; k_exact <= 0 switches it
         off.  Returns the number of loops that use it."""
         k_rule = k_exact if k_rule is None else k_rule
         n = _i(0)
         self.ctx.check(lib().disco_hip_plan_set_cubature(self._h, float(k_exact), float(k_rule),
                                                          ctypes.byref(n)))
+        return n.value
+
+    def set_band(self, k_band: float) -> int:
+        """Band form of a cubature plan: the two-ring cubature in place of the
+        exact field for lanes beyond k_band loop radii of a loop image (loops it
+        represents); k_band <= 0 switches it off.  Returns the number of loops
+        that use it."""
+        n = _i(0)
+        self.ctx.check(lib().disco_hip_plan_set_band(self._h, float(k_band), ctypes.byref(n)))
         return n.value
 
     def profile(self, on: bool = True):

@@ -187,7 +187,9 @@ __device__ __forceinline__ bool image_out(const LineLoop& L, int di, int dj, int
 // (rule_ok), each lane weighs the exact field by 1 - w and the cubature's by w,
 // w = rule_window(|q|^2) (0 near the loop, 1 far from it); a wave evaluates the
 // exact field only when one of its lanes (outside the near slab) has w < 1,
-// and the cubature only when one has w > 0.
+// and the cubature only when one has w > 0.  With the band form on
+// (disco_hip_plan_set_band), the two-ring cubature takes the exact field's
+// place for lanes with |q|^2 >= rtb.
 template <int KIND, bool CUBATURE>
 __device__ __forceinline__ void far_flat_loop(const FrameAtom& f, const LineLoop& L,
                                               const SegP* __restrict__ s, int img, int inner,
@@ -206,8 +208,19 @@ __device__ __forceinline__ void far_flat_loop(const FrameAtom& f, const LineLoop
                 } else {
                     const float t = g1 * g1 + (g2 * g2 + g3 * g3);
                     const float w = rule_window(t, L);
-                    if (DevMath::any(!near && w < 1.0f))
+                    // the two-ring rule stands in for the exact field beyond rtb
+                    // (never inside the near slab; rtb is huge when it is off)
+                    const bool band = t >= L.rtb;
+                    if (DevMath::any(!near && w < 1.0f && !band))
                         line_image_flat_f32<DevMath, KIND>(g1, g2, g3, s, L.nv, fo, o1, o2, o3);
+                    if (DevMath::any(w < 1.0f && band)) {
+                        float b1, b2, b3;
+                        band_image_f32<DevMath>(g1, g2, g3, t, L, b1, b2, b3);
+                        fo = band ? 0.0f : fo;
+                        o1 = band ? b1 : o1;
+                        o2 = band ? b2 : o2;
+                        o3 = band ? b3 : o3;
+                    }
                     if (DevMath::any(w > 0.0f)) {
                         float c1, c2, c3;
                         rule9_image_f32<DevMath>(g1, g2, g3, t, L, c1, c2, c3);
@@ -1550,6 +1563,34 @@ int disco_hip_plan_set_cubature(disco_plan* plan, double k_exact, double k_rule,
     return DISCO_OK;
 }
 
+int disco_hip_plan_set_band(disco_plan* plan, double k_band, int* eligible)
+{
+    if (!plan) return DISCO_ERR_ARG;
+    disco_ctx* ctx = plan->ctx;
+    if (plan->kind != 0 || plan->prec != DISCO_PREC_FP32C)
+        return fail(ctx, DISCO_ERR_ARG, "the band form is for FP32C line plans");
+    if (!(k_band <= 0.0) && !(k_band >= 2.5 && std::isfinite(k_band)))   // NaN included
+        return fail(ctx, DISCO_ERR_ARG, "band radius: need k_band >= 2.5");
+    int ok = 0;
+    for (LineLoop& L : plan->hloops) {
+        L.rtb = 3.0e38f;
+        if (k_band <= 0.0 || !L.band_ok) continue;
+        ok++;
+        const double R = L.rR;
+        // no lane of the near slab takes the band rule
+        L.rtb = (float)std::fmax(k_band * k_band * R * R, 1.0001 * (L.rn2 + L.hn * L.hn));
+    }
+    DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (!plan->hloops.empty()) {
+        DISCO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        DISCO_HIP_TRY(ctx, hipMemcpy(plan->d_loops, plan->hloops.data(),
+                                     plan->hloops.size() * sizeof(LineLoop),
+                                     hipMemcpyHostToDevice));
+    }
+    if (eligible) *eligible = k_band > 0.0 ? ok : 0;
+    return DISCO_OK;
+}
+
 int disco_hip_plan_set_image_skip(disco_plan* plan, const unsigned* skip, int nloops)
 {
     if (!plan) return DISCO_ERR_ARG;
@@ -1586,6 +1627,27 @@ int disco_hip_line_cubature(const disco_line_loop* loops, int nloops, double* ou
         o[2] = L.rw1;
         o[3] = L.rrho;
         o[4] = L.rR;
+    }
+    return DISCO_OK;
+}
+
+int disco_hip_line_band(const disco_line_loop* loops, int nloops, double* out)
+{
+    if (nloops < 0 || (nloops > 0 && (!loops || !out))) return DISCO_ERR_ARG;
+    for (int l = 0; l < nloops; l++) {
+        LinePack pk;
+        const std::string e = pack_line(loops + l, 1, 1.0, 1.0, 1.0, pk);
+        if (!e.empty()) return DISCO_ERR_ARG;
+        double* o = out + 6 * (size_t)l;
+        for (int i = 0; i < 6; i++) o[i] = 0.0;
+        if (pk.loops.empty()) continue;
+        const LineLoop& L = pk.loops[0];
+        o[0] = L.band_ok;
+        o[1] = L.bw0;
+        o[2] = L.bw1;
+        o[3] = L.bw2;
+        o[4] = L.brho1;
+        o[5] = L.brho2;
     }
     return DISCO_OK;
 }

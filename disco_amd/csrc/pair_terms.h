@@ -107,9 +107,19 @@ struct alignas(64) LineLoop {
     int src;                // index of the loop in the caller's loop array
     unsigned img_skip;      // images (i, j, k) in [-1, 1]^3 left out of the sum: bit
                             // 9 (i + 1) + 3 (j + 1) + (k + 1) (the caller sums them)
-    int rpad[11];
+    // Two-ring cubature of the cut (the band form of a cubature plan): the
+    // centre (area bw0) and two rings of twelve points (areas bw1, bw2 each) at
+    // radii brho1, brho2, ring 1 at angles j pi/6 from e1, ring 2 at
+    // j pi/6 + pi/12; it stands in for the exact field of lanes with
+    // r^2 >= rtb (a huge rtb: off)
+    float bw0, bw1, bw2;
+    float brho1, brho2;
+    float bmp1[6], bmp2[6]; // c2 m . P for the six pair directions of each ring
+    float rtb;
+    int band_ok;            // the two-ring rule matches the cut's moments through degree 9
+    int rpad2[8];
 };
-static_assert(sizeof(LineLoop) == 448, "loop record size");
+static_assert(sizeof(LineLoop) == 512, "loop record size");
 
 // One loop of a loopon plan; loopons are stored relative to (cx, cy, cz).
 struct alignas(128) LooponLoop {
@@ -600,6 +610,65 @@ DISCO_HD void rule9_image_f32(float q1, float q2, float h, float t, const LineLo
     const float AC = w0 * c0 + w1 * ac;
     o1 = L.rbm * (L.rmf[0] * S2 + (q1 * AC - w1 * A1));
     o2 = L.rbm * (L.rmf[1] * S2 + (q2 * AC - w1 * A2));
+    o3 = L.rbm * ((kR12 * T1 + L.rmf[2] * S2) + h * AC);
+}
+
+// The two-ring cubature (LineLoop::bw0 ... bmp2) of the same Kroupa term: the
+// centre and rings of twelve at radii brho1 (angles j pi/6) and brho2
+// (j pi/6 + pi/12), each ring as six pairs +-P sharing their dot products, as
+// in rule9_image_f32.
+// cos and sin of the pair directions: ring 0 at j pi/6, ring 1 at j pi/6 + pi/12
+constexpr float kBandCos[2][6] = {
+    {1.0f, 0.866025403784438647f, 0.5f, 0.0f, -0.5f, -0.866025403784438647f},
+    {0.965925826289068287f, 0.707106781186547524f, 0.258819045102520762f,
+     -0.258819045102520762f, -0.707106781186547524f, -0.965925826289068287f}};
+constexpr float kBandSin[2][6] = {
+    {0.0f, 0.5f, 0.866025403784438647f, 1.0f, 0.866025403784438647f, 0.5f},
+    {0.258819045102520762f, 0.707106781186547524f, 0.965925826289068287f,
+     0.965925826289068287f, 0.707106781186547524f, 0.258819045102520762f}};
+
+template <class M, int RING>
+DISCO_HD void band_ring_f32(float q1, float q2, float t, float rho, const float* mp, float mq,
+                            float h, float cb, float& t1, float& s2, float& ac, float& A1,
+                            float& A2)
+{
+    const float base = t + rho * rho;
+    const float Q1 = (2.0f * rho) * q1, Q2 = (2.0f * rho) * q2;
+    float a1 = 0.f, a2 = 0.f;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        const float c = kBandCos[RING][j], sn = kBandSin[RING][j];
+        float dc;
+        rule_ring_pair<M>(base, c * Q1 + sn * Q2, mq, mp[j], h, cb, t1, s2, ac, dc);
+        a1 += dc * c;
+        a2 += dc * sn;
+    }
+    A1 = rho * a1;
+    A2 = rho * a2;
+}
+
+template <class M>
+DISCO_HD void band_image_f32(float q1, float q2, float h, float t, const LineLoop& L,
+                             float& o1, float& o2, float& o3)
+{
+    constexpr float kC1 = (float)(-kC3 * (1.0 - 2.0 * kNu));
+    constexpr float kR12 = (float)((1.0 - 2.0 * kNu) / 3.0);     // c1 / c2
+    const float mq = L.rm[0] * q1 + (L.rm[1] * q2 + L.rm[2] * h);
+    const float cb = L.rcb;
+    const float i0 = M::rsqrt_f(t);
+    const float i02 = i0 * i0, i03 = i02 * i0;
+    const float c0 = i03 * ((h * mq) * i02 - cb);
+    float t1a = 0.f, s2a = 0.f, aca = 0.f, A1a, A2a;
+    float t1b = 0.f, s2b = 0.f, acb = 0.f, A1b, A2b;
+    band_ring_f32<M, 0>(q1, q2, t, L.brho1, L.bmp1, mq, h, cb, t1a, s2a, aca, A1a, A2a);
+    band_ring_f32<M, 1>(q1, q2, t, L.brho2, L.bmp2, mq, h, cb, t1b, s2b, acb, A1b, A2b);
+    const float w0 = L.bw0, w1 = L.bw1, w2 = L.bw2;
+    const float T1 = w0 * (i03 * mq) + (w1 * t1a + w2 * t1b);
+    const float S2 = kC1 * (h * (w0 * i03 + (w1 * s2a + w2 * s2b)));
+    const float AC = w0 * c0 + (w1 * aca + w2 * acb);
+    const float A1 = w1 * A1a + w2 * A1b, A2 = w1 * A2a + w2 * A2b;
+    o1 = L.rbm * (L.rmf[0] * S2 + (q1 * AC - A1));
+    o2 = L.rbm * (L.rmf[1] * S2 + (q2 * AC - A2));
     o3 = L.rbm * ((kR12 * T1 + L.rmf[2] * S2) + h * AC);
 }
 

@@ -130,14 +130,21 @@ class CubatureSum:
     as cubatures on a grid of spacing grid_h (default L/16).  Both grids are
     interpolated by 6-node (quintic) stencils (order = 4: cubic, with defaults
     L/64 and L/24).  Defaults: measured on the 4M-atom samples at images +-2
-    and +-4 (max |du| 1.0e-6 and 2.0e-6 A against the FP64 plain sum)."""
+    and +-4 (max |du| 1.0e-6 and 2.0e-6 A against the FP64 plain sum).
+
+    k_band (None: off): lanes between k_band and k_exact cut radii of a loop
+    image take the two-ring cubature (centre and two rings of twelve, error
+    O((R/r)^12)) in place of the exact line integral, for the loops that rule
+    represents (circles and regular polygons of twelve sides or more)."""
 
     def __init__(self, ctx, x, y, z, loops, box, img, k_exact=5.0, k_rule=None,
-                 grid_h=None, grid_groups=None, near_tol=1e-7, near_h=None, order=6):
+                 grid_h=None, grid_groups=None, near_tol=1e-7, near_h=None, order=6,
+                 k_band=None):
         self.ctx = ctx
         self.img = int(img)
         self.near = B.Plan(ctx, loops, box, min(1, self.img), B.PREC_FP32C)
         self.eligible = self.near.set_cubature(k_exact, k_rule)
+        self.band = self.near.set_band(k_band) if k_band else 0
         self.nloops = len(loops)
         self.grids = []
         self.skipped = 0
@@ -190,13 +197,13 @@ class CubatureSum:
 
 
 def cubature_sum_torch(ctx, x, y, z, loops, box, img, k_exact=5.0, k_rule=None, u=None,
-                       accumulate=False, grid_h=None, near_tol=1e-7, near_h=None):
+                       accumulate=False, grid_h=None, near_tol=1e-7, near_h=None, k_band=None):
     """One-shot CubatureSum: returns u (three device tensors)."""
     import torch
     if u is None:
         u = [torch.zeros_like(x) for _ in range(3)]
     cs = CubatureSum(ctx, x, y, z, loops, box, img, k_exact, k_rule, grid_h, near_tol=near_tol,
-                     near_h=near_h)
+                     near_h=near_h, k_band=k_band)
     cs.run(x, y, z, *u, accumulate=accumulate)
     cs.free()
     return u

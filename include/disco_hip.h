@@ -184,6 +184,24 @@ int disco_hip_plan_set_inner(disco_plan *plan, int inner);
  * 2 <= k_exact <= k_rule.  eligible (may be NULL) receives the number of loops
  * that use it; the others stay exact.  Synchronous. */
 int disco_hip_plan_set_cubature(disco_plan *plan, double k_exact, double k_rule, int *eligible);
+/* (No counterpart in the reference.)  Band form of a cubature plan
+ * (disco_hip_plan_set_cubature must be on for it to act): for each loop whose
+ * cut a two-ring cubature represents (the centre and two rings of twelve
+ * points, a Gauss-Radau rule in r^2 matching the cut's radial moments through
+ * I8; used only where it matches every moment of degree <= 9 and the cut's
+ * moments of degrees 10 and 11 are isotropic -- circles and regular polygons
+ * of twelve sides or more), a lane at distance r >= k_band R of a loop image
+ * takes that cubature in place of the exact field (the near slab is always
+ * exact; beyond k_exact R the 9-point cubature applies as before).  Its error
+ * falls as (R/r)^12: for |b| = 2.56 Angstrom at most ~1.2e-7 Angstrom at
+ * r = 3R.  k_band <= 0 switches it off (the default); otherwise
+ * k_band >= 2.5.  eligible (may be NULL) receives the number of loops that use
+ * it.  Synchronous. */
+int disco_hip_plan_set_band(disco_plan *plan, double k_band, int *eligible);
+/* Host only: for each loop, {eligible (0/1), centre weight, per-point weight
+ * of ring 1, of ring 2, ring radius 1, ring radius 2} of its two-ring cubature
+ * (out: 6 doubles per loop). */
+int disco_hip_line_band(const disco_line_loop *loops, int nloops, double *out);
 /* (No counterpart in the reference.)  Leave images of the +-1 shell out of a
  * line plan's sums: bit 9 (i + 1) + 3 (j + 1) + (k + 1) of skip[l] drops image
  * (i, j, k) of input loop l from every pass (the caller sums it another way,

@@ -38,15 +38,17 @@ def _err(a, b):
     return max(float((s - t).abs().max()) for s, t in zip(a, b))
 
 
-def test_c2_every_atom(hip_ctx):
-    """4,000,000 atoms, 50 loops, images +-2: the cubature form within 1e-5 A of
-    the FP64 plain sum on every atom."""
+@pytest.mark.parametrize("k_band", [None, 3.0])
+def test_c2_every_atom(hip_ctx, k_band):
+    """4,000,000 atoms, 50 loops, images +-2: the cubature form (and with the
+    two-ring band from 3 R) within 1e-5 A of the FP64 plain sum on every atom."""
     s = G.config_c2()
     x, y, z = _dev(s.x, s.y, s.z)
     ref = _plain64(hip_ctx, x, y, z, s.loops, s.box, s.img)
-    u = C.cubature_sum_torch(hip_ctx, x, y, z, s.loops, s.box, s.img, K_EXACT, near_tol=NEAR_TOL)
+    u = C.cubature_sum_torch(hip_ctx, x, y, z, s.loops, s.box, s.img, K_EXACT, near_tol=NEAR_TOL,
+                             k_band=k_band)
     e = _err(u, ref)
-    print(f"C2 cubature vs FP64 plain, all {s.n} atoms: {e:.3e} A")
+    print(f"C2 cubature (band {k_band}) vs FP64 plain, all {s.n} atoms: {e:.3e} A")
     assert e < TOL
 
 
@@ -67,10 +69,12 @@ def _near_cut_atoms(loops, rng, per_loop, dist):
     return np.concatenate(pts)
 
 
-def test_c5_images_4(hip_ctx):
+@pytest.mark.parametrize("k_band", [None, 3.0])
+def test_c5_images_4(hip_ctx, k_band):
     """200 loops drawn as the 4M-atom sample's, images +-4: 4,096 random atoms of
     the box and 4,000 atoms within 6 A of the loop cuts (the near slab, where the
-    sum is exact in FP64) against the FP64 plain sum over all 729 images."""
+    sum is exact in FP64) against the FP64 plain sum over all 729 images; the
+    cubature form and its two-ring band from 3 R."""
     s = G.config_c2(seed=20245, nloops=200)
     rng = np.random.default_rng(4)
     L = s.box[0]
@@ -79,15 +83,17 @@ def test_c5_images_4(hip_ctx):
     P = np.clip(P, -L / 2, L / 2 - 1e-9)          # inside the periodic box
     x, y, z = _dev(P[:, 0], P[:, 1], P[:, 2])
     ref = _plain64(hip_ctx, x, y, z, s.loops, s.box, 4)
-    cs = C.CubatureSum(hip_ctx, x, y, z, s.loops, s.box, 4, K_EXACT, near_tol=NEAR_TOL)
+    cs = C.CubatureSum(hip_ctx, x, y, z, s.loops, s.box, 4, K_EXACT, near_tol=NEAR_TOL,
+                       k_band=k_band)
     import torch
     u = [torch.zeros_like(x) for _ in range(3)]
     cs.run(x, y, z, *u)
     torch.cuda.synchronize()
     assert cs.eligible == 200 and cs.skipped > 0
+    assert cs.band == (0 if k_band is None else sum(len(lp.vx) >= 12 for lp in s.loops))
     cs.free()
     e = _err(u, ref)
-    print(f"C5 cubature vs FP64 plain (+-4), {x.numel()} atoms: {e:.3e} A")
+    print(f"C5 cubature (band {k_band}) vs FP64 plain (+-4), {x.numel()} atoms: {e:.3e} A")
     assert e < TOL
 
 
@@ -129,11 +135,12 @@ def test_image_skip_leaves_exactly_those_images_out(hip_ctx):
     q.free()
 
 
-@pytest.mark.parametrize("k_rule", [None, 8.0])
-def test_loop_forms_and_window(hip_ctx, k_rule):
+@pytest.mark.parametrize("k_rule,k_band", [(None, None), (8.0, None), (None, 3.0), (8.0, 3.0)])
+def test_loop_forms_and_window(hip_ctx, k_rule, k_band):
     """Glide, prismatic and general-b circles (the rule represents them), loops
     off their plane (summed in global axes, exact) and rectangles (anisotropic:
-    exact), images +-3 in a small box; the hard switch and a blend 5 R .. 8 R."""
+    exact), images +-3 in a small box; the hard switch and a blend 5 R .. 8 R,
+    each with and without the two-ring band from 3 R."""
     import torch
     rng = np.random.default_rng(21)
     loops = (kind_loops("glide", rng, 2) + kind_loops("prism", rng, 2) +
@@ -144,14 +151,16 @@ def test_loop_forms_and_window(hip_ctx, k_rule):
                             cy=80.0, cz=90.0))
     x, y, z = _dev(*atoms_near(loops, rng, nrand=3000, per_loop=100))
     ref = _plain64(hip_ctx, x, y, z, loops, BOX, 3)
-    cs = C.CubatureSum(hip_ctx, x, y, z, loops, BOX, 3, K_EXACT, k_rule, near_tol=NEAR_TOL)
+    cs = C.CubatureSum(hip_ctx, x, y, z, loops, BOX, 3, K_EXACT, k_rule, near_tol=NEAR_TOL,
+                       k_band=k_band)
     assert cs.eligible == 6                       # not the bent loop, not the rectangle
+    assert cs.band == (0 if k_band is None else 6)
     u = [torch.zeros_like(x) for _ in range(3)]
     cs.run(x, y, z, *u)
     torch.cuda.synchronize()
     cs.free()
     e = _err(u, ref)
-    print(f"loop forms, k_rule {k_rule}: {e:.3e} A")
+    print(f"loop forms, k_rule {k_rule}, band {k_band}: {e:.3e} A")
     assert e < TOL
 
 
@@ -172,11 +181,88 @@ def test_cubature_off_is_the_plain_fp32c_pass_and_errors(hip_ctx):
     for bad in ((1.5, 2.0), (5.0, 4.0), (5.0, float("inf"))):
         with pytest.raises(B.DiscoError):
             p.set_cubature(*bad)
+    # the band form: off by default and when switched off again
+    assert p.set_band(3.0) == 2 and p.set_band(0.0) == 0
+    c = [torch.zeros_like(x) for _ in range(3)]
+    p.run_torch(x, y, z, *c)
+    torch.cuda.synchronize()
+    assert all(torch.equal(s_, t) for s_, t in zip(a, c))
+    for bad in (2.0, float("inf"), float("nan")):
+        with pytest.raises(B.DiscoError):
+            p.set_band(bad)
     p.free()
     p64 = B.Plan(hip_ctx, s.loops, s.box, 1, B.PREC_FP64)
     with pytest.raises(B.DiscoError):
         p64.set_cubature(5.0)
+    with pytest.raises(B.DiscoError):
+        p64.set_band(3.0)
     p64.free()
+
+
+def _band_loop(kind, R, nseg, c, rng):
+    """A regular polygon of the given kind whose vertex 0 lies 1e-7 R farther out
+    than the others (so the packer's frame axis e1 points at it; the moments
+    stay isotropic far below the rule's eligibility bound)."""
+    from loop_kinds import _frame
+    n, e1, e2 = _frame(rng)
+    t = 2 * np.pi * np.arange(nseg) / nseg
+    rr = np.full(nseg, R)
+    rr[0] *= 1 + 1e-7
+    V = c + rr[:, None] * (np.cos(t)[:, None] * e1 + np.sin(t)[:, None] * e2)
+    b = {"glide": 2.5 * (0.6 * e1 + 0.8 * e2), "prism": 2.5 * n,
+         "mixed": 1.5 * n + 2.0 * e1 - 0.7 * e2}[kind]
+    nb = float(np.linalg.norm(b))
+    lp = B.LoopData(phys=(nb, nb, nb, *(b / nb), *n, 0.0), line=True, vx=V[:, 0].copy(),
+                    vy=V[:, 1].copy(), vz=V[:, 2].copy(), cx=c[0], cy=c[1], cz=c[2])
+    return lp, n, e1, e2, b
+
+
+@pytest.mark.parametrize("kind", ["glide", "prism", "mixed"])
+def test_band_kernel_is_the_two_ring_cubature(hip_ctx, kind):
+    """One loop image (images 0): lanes between 2.5 R and 5 R take the two-ring
+    cubature -- equal to the FP64 oracle's sum of its 25 points as loopons to
+    FP32 accuracy, and off the exact line integral by the rule's own error
+    (about 1e-6 A just beyond 2.5 R) --, lanes nearer the loop the exact field."""
+    import torch
+    import oracle
+    rng = np.random.default_rng({"glide": 31, "prism": 32, "mixed": 33}[kind])
+    R, nseg = {"glide": (30.0, 48), "prism": (20.0, 24), "mixed": (25.0, 36)}[kind]
+    c = np.array([500.0, 600.0, 700.0])
+    lp, n, e1, e2, b = _band_loop(kind, R, nseg, c, rng)
+    box = (2000.0, 2000.0, 2000.0)
+    d = rng.normal(size=(2 * 1500, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    r = np.concatenate([rng.uniform(1.6, 2.45, 1500), rng.uniform(2.55, 4.9, 1500)]) * R
+    P = c + r[:, None] * d
+    x, y, z = _dev(P[:, 0], P[:, 1], P[:, 2])
+    p = B.Plan(hip_ctx, [lp], box, 0, B.PREC_FP32C)
+    assert p.set_cubature(K_EXACT) == 1 and p.set_band(2.5) == 1
+    u = [torch.zeros_like(x) for _ in range(3)]
+    p.run_torch(x, y, z, *u)
+    torch.cuda.synchronize()
+    p.free()
+    got = np.stack([t.cpu().numpy() for t in u], 1)
+    ex = np.stack(oracle.line_sum(P[:, 0], P[:, 1], P[:, 2], [lp], box, 0), 1)
+    info = B.line_band([lp])[0]
+    nb = float(np.linalg.norm(b))
+    pts = []
+    for rho, w, npt, ph in ((0.0, info[1], 1, 0.0), (info[4], info[2], 12, 0.0),
+                            (info[5], info[3], 12, np.pi / 12)):
+        th = ph + 2 * np.pi * np.arange(npt) / npt
+        Q = c + rho * (np.cos(th)[:, None] * e1 + np.sin(th)[:, None] * e2)
+        pts.append(B.LoopData(phys=(nb, nb, nb, *(b / nb), *n, w), line=False, x=Q[:, 0].copy(),
+                              y=Q[:, 1].copy(), z=Q[:, 2].copy()))
+    rule = np.stack(oracle.loopon_sum(P[:, 0], P[:, 1], P[:, 2], pts, box, 0), 1)
+    inner, outer = slice(0, 1500), slice(1500, None)
+    e_in = np.abs(got[inner] - ex[inner]).max()
+    e_rule = np.abs(got[outer] - rule[outer]).max()
+    e_out = np.abs(got[outer] - ex[outer]).max()
+    scale = np.abs(rule[outer]).max()
+    print(f"{kind}: exact lanes vs line {e_in:.2e}, band lanes vs rule {e_rule:.2e} "
+          f"(field up to {scale:.2e}), vs line {e_out:.2e} A")
+    assert e_in < 5e-7
+    assert e_rule < 1e-6 * scale + 1e-9
+    assert 2e-7 < e_out < 3e-6
 
 
 def test_polylines_off_the_plane_in_cubature_form(hip_ctx):
