@@ -51,16 +51,22 @@ from . import surrogate as S
 GRID_ERR = {4: 3 * 0.31 * 120 * (9 / 16) / 24, 6: 3 * 0.31 * 5040 * (225 / 64) / 720}
 
 
-def far_images(loops, box, lo, hi, h, tol, k_exact, dmin=0.0, info=None, order=6):
+def far_images(loops, box, lo, hi, h, tol, k_exact, dmin=0.0, info=None, order=6, k_band=None,
+               band=None):
     """Images of the +-1 shell to sum on a grid of spacing h instead of at the
     atoms (loops whose cut the cubature represents only): those whose centre
     lies at least max(k_exact R, R + dmin, (GRID_ERR |b| R^2 h^p / tol)^(1/(p+2)))
     (p = order) from the box [lo, hi] holding the atoms, so that the cubature
     alone is accurate at every atom and the grid's interpolation error is below
-    tol.
+    tol.  With k_band (and band = B.line_band(loops)), an image of a loop the
+    two-ring cubature represents also goes to the grid when its centre lies at
+    least max(k_band R + 3 h, R + dmin, the interpolation bound) from the box:
+    every node an atom's stencil reaches (within 3 h of the box) is then at
+    least k_band R from the image's centre, where that cubature is accurate.
     Returns the skip mask of each loop (bit 9 (i + 1) + 3 (j + 1) + (k + 1))
-    and the list of (loop index, offset) of those images, offset =
-    (i Lx, j Ly, k Lz) (the image of a loop centred at c sits at c + offset)."""
+    and the list of (loop index, offset, two_ring) of those images, offset =
+    (i Lx, j Ly, k Lz) (the image of a loop centred at c sits at c + offset),
+    two_ring true when the grid must sum the two-ring cubature there."""
     info = B.line_cubature(loops) if info is None else info
     lo, hi = np.asarray(lo, float), np.asarray(hi, float)
     masks = np.zeros(len(loops), np.uint32)
@@ -71,18 +77,56 @@ def far_images(loops, box, lo, hi, h, tol, k_exact, dmin=0.0, info=None, order=6
         c = np.array([lp.cx, lp.cy, lp.cz])
         R = float(info[l, 4])
         bm = float(np.linalg.norm(lp.burgers()))
-        dmin_l = max(k_exact * R, R + dmin,
-                     (GRID_ERR[order] * bm * R * R * h ** order / tol) ** (1 / (order + 2)))
+        interp = (GRID_ERR[order] * bm * R * R * h ** order / tol) ** (1 / (order + 2))
+        dmin_l = max(k_exact * R, R + dmin, interp)
+        dmin_b = np.inf
+        if k_band is not None and band is not None and band[l, 0]:
+            dmin_b = max(k_band * R + 3.0 * h, R + dmin, interp)
         for i, j, k in itertools.product((-1, 0, 1), repeat=3):
             if i == j == k == 0:
                 continue
             off = np.array([i * box[0], j * box[1], k * box[2]])
             ci = c + off
             gap = np.maximum(np.maximum(lo - ci, ci - hi), 0.0)
-            if float(np.sqrt((gap * gap).sum())) >= dmin_l:
+            d = float(np.sqrt((gap * gap).sum()))
+            if d >= dmin_l or d >= dmin_b:
                 masks[l] |= np.uint32(1 << (9 * (i + 1) + 3 * (j + 1) + (k + 1)))
-                out.append((l, off))
+                out.append((l, off, d < dmin_l))
     return masks, out
+
+
+def band_loopons(loops, band):
+    """Per loop, the loopons of its two-ring cubature ([centre, ring 1, ring 2];
+    None where band[l, 0] is 0): weights and radii from B.line_band, ring 1 at
+    angles j pi/6 from the in-plane direction of the loop's first vertex, ring 2
+    at j pi/6 + pi/12 (the rule is isotropic through rank 11, so the rings'
+    orientation enters at degree 12 only)."""
+    out = [None] * len(loops)
+    th = 2.0 * np.pi * np.arange(12) / 12
+    for l, lp in enumerate(loops):
+        if not band[l, 0]:
+            continue
+        c = np.array([lp.cx, lp.cy, lp.cz])
+        V = np.stack([lp.vx, lp.vy, lp.vz], 1) - c
+        N = np.cross(V, np.roll(V, -1, axis=0)).sum(0)
+        n = N / np.linalg.norm(N)
+        e1 = V[0] - (V[0] @ n) * n
+        e1 /= np.linalg.norm(e1)
+        e2 = np.cross(n, e1)
+        b = np.array(lp.burgers())
+        bm = float(np.linalg.norm(b))
+        m = b / bm
+        parts = []
+        for rho, w, ph in ((0.0, band[l, 1], None), (band[l, 4], band[l, 2], 0.0),
+                           (band[l, 5], band[l, 3], np.pi / 12)):
+            if ph is None:
+                P = c[None]
+            else:
+                P = c + rho * (np.cos(th + ph)[:, None] * e1 + np.sin(th + ph)[:, None] * e2)
+            parts.append(B.LoopData(phys=(bm, bm, bm, *m, *n, float(w)), line=False,
+                                    x=P[:, 0].copy(), y=P[:, 1].copy(), z=P[:, 2].copy()))
+        out[l] = parts
+    return out
 
 
 class _Grid:
@@ -135,7 +179,10 @@ class CubatureSum:
     k_band (None: off): lanes between k_band and k_exact cut radii of a loop
     image take the two-ring cubature (centre and two rings of twelve, error
     O((R/r)^12)) in place of the exact line integral, for the loops that rule
-    represents (circles and regular polygons of twelve sides or more)."""
+    represents (circles and regular polygons of twelve sides or more); the
+    +-1 images at least k_band R + 3 near_h from the atoms (and beyond the
+    interpolation bound) then go to the near_h grid with that cubature
+    (far_images)."""
 
     def __init__(self, ctx, x, y, z, loops, box, img, k_exact=5.0, k_rule=None,
                  grid_h=None, grid_groups=None, near_tol=1e-7, near_h=None, order=6,
@@ -148,6 +195,7 @@ class CubatureSum:
         self.nloops = len(loops)
         self.grids = []
         self.skipped = 0
+        self.skipped_band = 0
         hf = float(grid_h) if grid_h else min(box) / (16.0 if order == 6 else 24.0)
         parts = {}                 # grid spacing -> [(loopon loops, img, inner)]
         base = None
@@ -159,21 +207,27 @@ class CubatureSum:
             for l, lp in enumerate(loops):      # the grid sums the ring rule's cubature
                 if info[l, 0] and not S.ring_rule_ok(lp):
                     info[l, 0] = 0.0
+            band = None
+            if k_band:
+                band = B.line_band(loops)
+                band[:, 0] *= info[:, 0]        # the grid needs the ring rule's loops
             masks, far = far_images(loops, box, lo, hi, hn, float(near_tol),
                                     max(float(k_exact), float(k_rule or 0.0)), info=info,
-                                    order=order)
+                                    order=order, k_band=k_band, band=band)
             if far:
                 ctx.check(B.lib().disco_hip_plan_set_image_skip(
                     self.near._h, masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
                     len(loops)))
                 base = S.rule_loopons_per_loop(loops, ring=8)
+                two = band_loopons(loops, band) if band is not None else None
                 copies = []
-                for l, off in far:
-                    for part in base[l]:
+                for l, off, two_ring in far:
+                    for part in (two[l] if two_ring else base[l]):
                         copies.append(B.LoopData(phys=part.phys, line=False, x=part.x + off[0],
                                                  y=part.y + off[1], z=part.z + off[2]))
                 parts.setdefault(hn, []).append((copies, 0, -1))
                 self.skipped = len(far)
+                self.skipped_band = sum(1 for f in far if f[2])
         if self.img > 1:
             base = S.rule_loopons_per_loop(loops, ring=8) if base is None else base
             parts.setdefault(hf, []).append(([p for ps in base for p in ps], self.img, 1))

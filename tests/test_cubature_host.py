@@ -76,7 +76,8 @@ def test_far_image_selection():
     lo, hi = (-50.0, -50.0, -50.0), (50.0, 50.0, 50.0)
     h, tol, k = 2.0, 1e-7, 6.0
     masks, far = C.far_images([lp], box, lo, hi, h, tol, k, dmin=30.0, order=4)
-    got = {tuple(int(round(t / 100.0)) for t in off) for _, off in far}
+    got = {tuple(int(round(t / 100.0)) for t in off) for _, off, _ in far}
+    assert not any(two for _, _, two in far)
     R = C.B.line_cubature([lp])[0, 4]
     dmin = max(k * R, R + 30.0, (C.GRID_ERR[4] * 2.5 * R * R * h ** 4 / tol) ** (1 / 6))
     want = set()
@@ -102,6 +103,52 @@ def test_far_image_selection():
     # loops the rule does not represent are never taken out
     m2, f2 = C.far_images([_rect(5, 3)], box, lo, hi, h, tol, k)
     assert int(m2[0]) == 0 and not f2
+    # with the two-ring band from 2.5 R (grid spacing 1, where k R binds), images
+    # at max(2.5 R + 3 h, R + dmin, the interpolation bound) .. dmin join them,
+    # marked for the two-ring cubature
+    h1 = 1.0
+    m1, f1 = C.far_images([lp], box, lo, hi, h1, tol, k, dmin=30.0, order=4)
+    band = C.B.line_band([lp])
+    mb, fb = C.far_images([lp], box, lo, hi, h1, tol, k, dmin=30.0, order=4, k_band=2.5,
+                          band=band)
+    interp = (C.GRID_ERR[4] * 2.5 * R * R * h1 ** 4 / tol) ** (1 / 6)
+    dmin1 = max(k * R, R + 30.0, interp)
+    dmin_b = max(2.5 * R + 3 * h1, R + 30.0, interp)
+    assert dmin_b < dmin1
+    for _, off, two in fb:
+        i, j, kk = (int(round(t / 100.0)) for t in off)
+        ci = np.array([40.0 + 100 * i, 100.0 * j, 100.0 * kk])
+        d = np.linalg.norm(np.maximum(np.maximum(np.array(lo) - ci, ci - np.array(hi)), 0))
+        assert d >= dmin_b and two == (d < dmin1)
+    assert len(fb) > len(f1) and any(two for _, _, two in fb)
+    assert {o.tobytes() for _, o, t in fb if not t} == {o.tobytes() for _, o, _ in f1}
+    # no band where the loop's cut is not eligible
+    mr, fr = C.far_images([_rect(5, 3)], box, lo, hi, h, tol, k, k_band=2.5,
+                          band=C.B.line_band([_rect(5, 3)]))
+    assert int(mr[0]) == 0 and not fr
+
+
+def test_band_loopons_are_the_two_ring_rule():
+    """The grid's loopons of the two-ring cubature: the centre and rings of
+    twelve at the library's radii and per-point weights, in the loop's plane,
+    ring 2 turned by pi/12 from ring 1; total area the cut's."""
+    lp = _polygon(48, 12.0, c=(3.0, 4.0, 5.0), n=(0.0, 0.6, 0.8), e1=(1.0, 0.0, 0.0))
+    band = C.B.line_band([lp])
+    cen, r1, r2 = C.band_loopons([lp], band)[0]
+    c = np.array([3.0, 4.0, 5.0])
+    n = np.array([0.0, 0.6, 0.8])
+    assert np.allclose([cen.x[0], cen.y[0], cen.z[0]], c)
+    for part, rho in ((r1, band[0, 4]), (r2, band[0, 5])):
+        P = np.stack([part.x, part.y, part.z], 1) - c
+        assert P.shape == (12, 3)
+        assert np.allclose(np.linalg.norm(P, axis=1), rho) and np.allclose(P @ n, 0.0, atol=1e-12)
+    u1 = np.array([r1.x[0], r1.y[0], r1.z[0]]) - c
+    u2 = np.array([r2.x[0], r2.y[0], r2.z[0]]) - c
+    cosang = (u1 @ u2) / (np.linalg.norm(u1) * np.linalg.norm(u2))
+    assert abs(cosang - np.cos(np.pi / 12)) < 1e-12
+    area = cen.phys[9] + 12 * (r1.phys[9] + r2.phys[9])
+    A = 0.5 * 48 * 144.0 * np.sin(2 * np.pi / 48)
+    assert abs(area - A) < 1e-5 * A
 
 
 # ------------------------------------------------------------ cuts that are not flat discs
