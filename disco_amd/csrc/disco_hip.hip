@@ -207,7 +207,11 @@ __device__ __forceinline__ void far_flat_loop(const FrameAtom& f, const LineLoop
                     line_image_flat_f32<DevMath, KIND>(g1, g2, g3, s, L.nv, fo, o1, o2, o3);
                 } else {
                     const float t = g1 * g1 + (g2 * g2 + g3 * g3);
-                    const float w = rule_window(t, L);
+                    // a hard switch at rt0 (k_rule = k_exact, rwinv huge): the
+                    // window is the step that rule_window gives there, and no
+                    // lane blends
+                    const bool hard = L.rwinv >= 1e29f;
+                    const float w = hard ? (t > L.rt0 ? 1.0f : 0.0f) : rule_window(t, L);
                     // the two-ring rule stands in for the exact field beyond rtb
                     // (never inside the near slab; rtb is huge when it is off)
                     const bool band = t >= L.rtb;
@@ -226,12 +230,20 @@ __device__ __forceinline__ void far_flat_loop(const FrameAtom& f, const LineLoop
                         rule9_image_f32<DevMath>(g1, g2, g3, t, L, c1, c2, c3);
                         // selects, so that a field not evaluated (or not finite
                         // where its weight is zero) never enters the sum
-                        const float we = 1.0f - w;
-                        const bool all_c = w >= 1.0f, mix = w > 0.0f && !all_c;
-                        fo = all_c ? 0.0f : (mix ? we * fo : fo);
-                        o1 = all_c ? c1 : (mix ? we * o1 + w * c1 : o1);
-                        o2 = all_c ? c2 : (mix ? we * o2 + w * c2 : o2);
-                        o3 = all_c ? c3 : (mix ? we * o3 + w * c3 : o3);
+                        const bool all_c = w >= 1.0f;
+                        if (hard) {
+                            fo = all_c ? 0.0f : fo;
+                            o1 = all_c ? c1 : o1;
+                            o2 = all_c ? c2 : o2;
+                            o3 = all_c ? c3 : o3;
+                        } else {
+                            const float we = 1.0f - w;
+                            const bool mix = w > 0.0f && !all_c;
+                            fo = all_c ? 0.0f : (mix ? we * fo : fo);
+                            o1 = all_c ? c1 : (mix ? we * o1 + w * c1 : o1);
+                            o2 = all_c ? c2 : (mix ? we * o2 + w * c2 : o2);
+                            o3 = all_c ? c3 : (mix ? we * o3 + w * c3 : o3);
+                        }
                     }
                 }
                 if (!near) {

@@ -561,7 +561,8 @@ DISCO_HD void line_image_flat_f32(float q1, float q2, float h, const SegP* __res
 // The ring points come in pairs +-P: |q -+ P|^2 = (|q|^2 + rho^2) -+ 2 q.P and
 // c2 m.(q -+ P) = c2 m.q -+ rmp, so a pair shares its dot products, and
 // sum co (r1, r2) = (q1, q2) sum co - sum co P: per pair only (co+ - co-) P.
-template <class M>
+// FIRST: the ring's first pair sets the sums instead of adding to them.
+template <class M, bool FIRST = false>
 DISCO_HD void rule_ring_pair(float base, float d, float mq, float mp, float h, float cb,
                              float& t1, float& s2, float& ac, float& dc)
 {
@@ -570,11 +571,12 @@ DISCO_HD void rule_ring_pair(float base, float d, float mq, float mp, float h, f
     const float ip2 = ip * ip, in2 = in * in;
     const float ip3 = ip2 * ip, in3 = in2 * in;
     const float mrp = mq - mp, mrn = mq + mp;
-    t1 += ip3 * mrp + in3 * mrn;
-    s2 += ip3 + in3;
+    const float dt = ip3 * mrp + in3 * mrn, ds = ip3 + in3;
     const float cp = ip3 * ((h * mrp) * ip2 - cb);
     const float cn = in3 * ((h * mrn) * in2 - cb);
-    ac += cp + cn;
+    t1 = FIRST ? dt : t1 + dt;
+    s2 = FIRST ? ds : s2 + ds;
+    ac = FIRST ? cp + cn : ac + (cp + cn);
     dc = cp - cn;
 }
 
@@ -596,8 +598,8 @@ DISCO_HD void rule9_image_f32(float q1, float q2, float h, float t, const LineLo
     const float base = t + rho * rho;
     const float u1 = (2.0f * rho) * q1, u2 = (2.0f * rho) * q2;   // 2 q.P on the axes
     const float v1 = (u1 + u2) * kS, v2 = (u2 - u1) * kS;          // and on the diagonals
-    float t1 = 0.f, s2 = 0.f, ac = 0.f, d0, d1, d2, d3;
-    rule_ring_pair<M>(base, u1, mq, L.rmp[0], h, cb, t1, s2, ac, d0);
+    float t1, s2, ac, d0, d1, d2, d3;
+    rule_ring_pair<M, true>(base, u1, mq, L.rmp[0], h, cb, t1, s2, ac, d0);
     rule_ring_pair<M>(base, u2, mq, L.rmp[1], h, cb, t1, s2, ac, d1);
     rule_ring_pair<M>(base, v1, mq, L.rmp[2], h, cb, t1, s2, ac, d2);
     rule_ring_pair<M>(base, v2, mq, L.rmp[3], h, cb, t1, s2, ac, d3);
@@ -634,9 +636,19 @@ DISCO_HD void band_ring_f32(float q1, float q2, float t, float rho, const float*
 {
     const float base = t + rho * rho;
     const float Q1 = (2.0f * rho) * q1, Q2 = (2.0f * rho) * q2;
-    float a1 = 0.f, a2 = 0.f;
+    float a1, a2, d0;
+    if (RING == 0) {                   // first direction (1, 0)
+        rule_ring_pair<M, true>(base, Q1, mq, mp[0], h, cb, t1, s2, ac, d0);
+        a1 = d0;
+        a2 = 0.0f;
+    } else {
+        const float c = kBandCos[1][0], sn = kBandSin[1][0];
+        rule_ring_pair<M, true>(base, c * Q1 + sn * Q2, mq, mp[0], h, cb, t1, s2, ac, d0);
+        a1 = d0 * c;
+        a2 = d0 * sn;
+    }
 #pragma unroll
-    for (int j = 0; j < 6; j++) {
+    for (int j = 1; j < 6; j++) {
         const float c = kBandCos[RING][j], sn = kBandSin[RING][j];
         float dc;
         rule_ring_pair<M>(base, c * Q1 + sn * Q2, mq, mp[j], h, cb, t1, s2, ac, dc);
@@ -658,8 +670,8 @@ DISCO_HD void band_image_f32(float q1, float q2, float h, float t, const LineLoo
     const float i0 = M::rsqrt_f(t);
     const float i02 = i0 * i0, i03 = i02 * i0;
     const float c0 = i03 * ((h * mq) * i02 - cb);
-    float t1a = 0.f, s2a = 0.f, aca = 0.f, A1a, A2a;
-    float t1b = 0.f, s2b = 0.f, acb = 0.f, A1b, A2b;
+    float t1a, s2a, aca, A1a, A2a;
+    float t1b, s2b, acb, A1b, A2b;
     band_ring_f32<M, 0>(q1, q2, t, L.brho1, L.bmp1, mq, h, cb, t1a, s2a, aca, A1a, A2a);
     band_ring_f32<M, 1>(q1, q2, t, L.brho2, L.bmp2, mq, h, cb, t1b, s2b, acb, A1b, A2b);
     const float w0 = L.bw0, w1 = L.bw1, w2 = L.bw2;
