@@ -99,14 +99,17 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kLooponChunk = 64;   // loopons per FP32 partial sum (flushed to FP64)
+constexpr double kMaxShiftBytes = 1073741824.0;   // a plan's image shift table
 
 // ------------------------------------------------------------------ per-atom sums
 // The lane in the frame (e1, e2, n) of loop L: Q = R (p - c), and an image's
-// offset Q - (di X + dj Y + dk Z) with the box edges X, Y, Z of the loop record
-// (wave-uniform, held in scalar registers).  Both line passes compute these
-// with contraction off, so the near predicate (|h| < hn and in-plane radius^2
-// < rn2) takes exactly the same decision in each and every (atom, loop, image)
-// is summed by exactly one of them.
+// offset Q - ((di X + dj Y) + dk Z) with the box edges X, Y, Z of the loop
+// record; the shift (di X + dj Y) + dk Z is wave-uniform and comes from the
+// plan's table in the FP64 segment table (pack_shifts: slot L.sh0 on, read with
+// scalar loads).  Every line pass takes it from there and Q with contraction
+// off, so the near predicate (|h| < hn and in-plane radius^2 < rn2) takes
+// exactly the same decision in each and every (atom, loop, image) is summed by
+// exactly one of them.
 struct FrameAtom {
     double Q1, Q2, Q3;
 };
@@ -120,15 +123,17 @@ __device__ __forceinline__ void frame_atom(double pcx, double pcy, double pcz, c
     f.Q3 = (pcx * L.nx + pcy * L.ny) + pcz * L.nz;
 }
 
-__device__ __forceinline__ bool frame_image(const FrameAtom& f, int di, int dj, int dk,
-                                            const LineLoop& L, double& q1, double& q2,
-                                            double& q3)
+__device__ __forceinline__ bool frame_image(const FrameAtom& f, int di, int dj, int dk, int img,
+                                            const LineLoop& L, const SegD* __restrict__ segd,
+                                            double& q1, double& q2, double& q3)
 {
 #pragma clang fp contract(off)
-    const double i = di, j = dj, k = dk;
-    q1 = f.Q1 - ((i * L.X1 + j * L.Y1) + k * L.Z1);
-    q2 = f.Q2 - ((i * L.X2 + j * L.Y2) + k * L.Z2);
-    q3 = f.Q3 - ((i * L.X3 + j * L.Y3) + k * L.Z3);
+    const int n = 2 * img + 1;
+    const double* D = reinterpret_cast<const double*>(segd + L.sh0) +
+                      4 * (((di + img) * n + (dj + img)) * n + (dk + img));
+    q1 = f.Q1 - D[0];
+    q2 = f.Q2 - D[1];
+    q3 = f.Q3 - D[2];
     return fabs(q3) < L.hn && q1 * q1 + q2 * q2 < L.rn2;
 }
 
@@ -192,7 +197,8 @@ __device__ __forceinline__ bool image_out(const LineLoop& L, int di, int dj, int
 // place for lanes with |q|^2 >= rtb.
 template <int KIND, bool CUBATURE>
 __device__ __forceinline__ void far_flat_loop(const FrameAtom& f, const LineLoop& L,
-                                              const SegP* __restrict__ s, int img, int inner,
+                                              const SegP* __restrict__ s,
+                                              const SegD* __restrict__ segd, int img, int inner,
                                               double& sx, double& sy, double& sz)
 {
     for (int di = -img; di <= img; di++)
@@ -200,7 +206,7 @@ __device__ __forceinline__ void far_flat_loop(const FrameAtom& f, const LineLoop
             for (int dk = -img; dk <= img; dk++) {
                 if (image_out(L, di, dj, dk, inner)) continue;
                 double q1, q2, q3;
-                const bool near = frame_image(f, di, dj, dk, L, q1, q2, q3);
+                const bool near = frame_image(f, di, dj, dk, img, L, segd, q1, q2, q3);
                 const float g1 = (float)q1, g2 = (float)q2, g3 = (float)q3;
                 float fo = 0.f, o1 = 0.f, o2 = 0.f, o3 = 0.f;
                 if (!CUBATURE || !L.rule_ok) {
@@ -278,7 +284,7 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
                         if (image_out(L, di, dj, dk, inner)) continue;
                         if (L.flat64) {
                             double q1, q2, q3;
-                            frame_image(f, di, dj, dk, L, q1, q2, q3);
+                            frame_image(f, di, dj, dk, img, L, segd, q1, q2, q3);
                             add_flat_f64(q1, q2, q3, L, segd, sx, sy, sz);
                             continue;
                         }
@@ -293,7 +299,7 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
         frame_atom(pcx, pcy, pcz, L, f);
         if (PASS == PASS_FAR && KIND != LOOP_3D) {
             const SegP* sp = reinterpret_cast<const SegP*>(segf + L.seg0);
-            far_flat_loop<KIND, CUBATURE>(f, L, sp, img, inner, sx, sy, sz);
+            far_flat_loop<KIND, CUBATURE>(f, L, sp, segd, img, inner, sx, sy, sz);
             continue;
         }
         if (PASS == PASS_NEAR) {
@@ -307,7 +313,7 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
                     for (int dk = k0; dk <= k1; dk++) {
                         if (image_out(L, di, dj, dk, inner)) continue;
                         double q1, q2, q3;
-                        if (!frame_image(f, di, dj, dk, L, q1, q2, q3)) continue;
+                        if (!frame_image(f, di, dj, dk, img, L, segd, q1, q2, q3)) continue;
                         any = true;
                         if (L.flat64) {
                             add_flat_f64(q1, q2, q3, L, segd, sx, sy, sz);
@@ -326,7 +332,7 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
                 for (int dk = -img; dk <= img; dk++) {
                     if (image_out(L, di, dj, dk, inner)) continue;
                     double q1, q2, q3;
-                    const bool near = frame_image(f, di, dj, dk, L, q1, q2, q3);
+                    const bool near = frame_image(f, di, dj, dk, img, L, segd, q1, q2, q3);
                     const double qx = pcx - di * Lx, qy = pcy - dj * Ly, qz = pcz - dk * Lz;
                     if (PASS == PASS_FAR) {
                         // LOOP_3D: the global-axes FP32 form
@@ -976,6 +982,11 @@ int make_line_plan(disco_ctx* ctx, const disco_line_loop* loops, int nloops, dou
     LinePack pk;
     const std::string e = pack_line(loops, nloops, Lx, Ly, Lz, pk);
     if (!e.empty()) return fail(ctx, DISCO_ERR_ARG, e);
+    const double nimg = (2.0 * img + 1) * (2.0 * img + 1) * (2.0 * img + 1);
+    if ((double)pk.loops.size() * nimg * 32.0 > kMaxShiftBytes)
+        return fail(ctx, DISCO_ERR_TOO_LARGE,
+                    "image shift table (32 B x loops x (2 img + 1)^3) above 1 GiB");
+    pack_shifts(pk, img);
     disco_plan* p = new (std::nothrow) disco_plan();
     if (!p) return fail(ctx, DISCO_ERR_OOM, "host allocation");
     p->ctx = ctx; p->kind = 0; p->prec = prec; p->img = img;
@@ -1706,6 +1717,7 @@ int disco_hip_batch_plan(disco_ctx* ctx, const disco_sample* samples, int nsampl
     b->ctx = ctx; b->prec = precision; b->img = img; b->nsamples = nsamples;
     LinePack all;
     const double im = (2.0 * img + 1) * (2.0 * img + 1) * (2.0 * img + 1);
+    double shift_bytes = 0.0;
     for (int k = 0; k < nsamples; k++) {
         const disco_sample& S = samples[k];
         std::string why;
@@ -1719,12 +1731,23 @@ int disco_hip_batch_plan(disco_ctx* ctx, const disco_sample* samples, int nsampl
             free_batch(b);
             return fail(ctx, DISCO_ERR_ARG, "sample " + std::to_string(k) + ": " + why);
         }
+        shift_bytes += (double)pk.loops.size() * im * 32.0;
+        if (shift_bytes > kMaxShiftBytes) {
+            free_batch(b);
+            return fail(ctx, DISCO_ERR_TOO_LARGE,
+                        "image shift table (32 B x loops x (2 img + 1)^3) above 1 GiB");
+        }
+        pack_shifts(pk, img);
         BatchSample bs;
         std::memset(&bs, 0, sizeof(bs));
         bs.a0 = S.atom_offset; bs.n = S.n;
         const int loop_base = (int)all.loops.size();
         const int seg_base = (int)all.segd.size();
-        for (LineLoop L : pk.loops) { L.seg0 += seg_base; all.loops.push_back(L); }
+        for (LineLoop L : pk.loops) {
+            L.seg0 += seg_base;
+            L.sh0 += seg_base;
+            all.loops.push_back(L);
+        }
         all.segd.insert(all.segd.end(), pk.segd.begin(), pk.segd.end());
         all.segf.insert(all.segf.end(), pk.segf.begin(), pk.segf.end());
         for (int q = 0; q < 5; q++) bs.kofs[q] = loop_base + pk.kofs[q];

@@ -117,7 +117,8 @@ struct alignas(64) LineLoop {
     float bmp1[6], bmp2[6]; // c2 m . P for the six pair directions of each ring
     float rtb;
     int band_ok;            // the two-ring rule matches the cut's moments through degree 9
-    int rpad2[8];
+    int sh0;                // first FP64-table slot of the loop's image shifts (plans)
+    int rpad2[7];
 };
 static_assert(sizeof(LineLoop) == 512, "loop record size");
 

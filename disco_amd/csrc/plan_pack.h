@@ -334,6 +334,45 @@ inline void pack_rule_band(const std::vector<double>& p1, const std::vector<doub
     L.band_ok = 1;
 }
 
+// Image shifts of every loop of a line plan in the loop's frame, appended to
+// the FP64 table as 128-B slots of four entries: entry
+// ((i + img) n + (j + img)) n + (k + img), n = 2 img + 1, of loop L holds
+// (i X + j Y) + k Z (three doubles and a pad) with the box edges X, Y, Z of its
+// record, computed with contraction off -- the expression the kernels would
+// otherwise evaluate per lane, so the image coordinates q = Q - shift come out
+// bit for bit the same.  L.sh0 is the loop's first slot.  The FP32 table gets
+// as many empty slots, so that the two tables keep one index (seg0) per
+// segment.  Returns the bytes added to the FP64 table.
+inline size_t pack_shifts(LinePack& pk, int img)
+{
+#pragma clang fp contract(off)
+    const int n = 2 * img + 1, nimg = n * n * n, slots = (nimg + 3) / 4;
+    std::vector<double> buf((size_t)slots * 16);
+    for (LineLoop& L : pk.loops) {
+        L.sh0 = (int)pk.segd.size();
+        std::fill(buf.begin(), buf.end(), 0.0);
+        for (int di = -img; di <= img; di++)
+            for (int dj = -img; dj <= img; dj++)
+                for (int dk = -img; dk <= img; dk++) {
+                    const double i = di, j = dj, k = dk;
+                    double* e = buf.data() + 4 * (((size_t)(di + img) * n + (dj + img)) * n +
+                                                  (dk + img));
+                    e[0] = (i * L.X1 + j * L.Y1) + k * L.Z1;
+                    e[1] = (i * L.X2 + j * L.Y2) + k * L.Z2;
+                    e[2] = (i * L.X3 + j * L.Y3) + k * L.Z3;
+                }
+        SegF empty;
+        std::memset(&empty, 0, sizeof(empty));
+        for (int t = 0; t < slots; t++) {
+            SegD d;
+            std::memcpy(&d, buf.data() + 16 * (size_t)t, sizeof(d));
+            pk.segd.push_back(d);
+            pk.segf.push_back(empty);
+        }
+    }
+    return pk.loops.size() * (size_t)slots * sizeof(SegD);
+}
+
 // Returns "" on success, else the reason the input is rejected.
 inline std::string pack_line(const disco_line_loop* in, int nloops, double Lx, double Ly,
                              double Lz, LinePack& out)
