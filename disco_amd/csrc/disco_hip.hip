@@ -454,13 +454,14 @@ __device__ __forceinline__ void atom_loopon(double px, double py, double pz,
 
 // ------------------------------------------------------------------ one-sample kernels
 // FP64 line sum (PASS_FP64) or the FP32C far pass over loops of kind KIND
-// (PASS_FAR): write u (or add).  At least 3 waves per SIMD: the FP64
-// loop-frame walk, two segments per iteration, fits 168 registers (at 4 waves
-// and 128 registers the walk needs one segment per iteration and register
-// copies of the carried terms: 2 % slower, the same bits); the FP32 ones use
-// about 70 and are unaffected.
+// (PASS_FAR): write u (or add).  Waves per SIMD: the FP64 loop-frame walk,
+// two segments per iteration, takes 5 (102 registers: the compiler then keeps
+// a few values of the rare branches and of the per-loop set-up in scratch,
+// none of the walk's; 1,490 ms against 1,510 at 4 waves and 1,561 at 3 on the
+// 4M-atom sample, the same bits); the global-axes FP64 walk keeps 3 (168
+// registers, no scratch); the FP32 ones use at most 74 registers either way.
 template <int PASS, int KIND, bool CUBATURE = false>
-__global__ __launch_bounds__(kBlock, 3) void line_kernel(
+__global__ __launch_bounds__(kBlock, (PASS == PASS_FP64 && KIND == FP64_FRAME) ? 5 : 3) void line_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
     const LineLoop* __restrict__ loops, int nloops, const SegF* __restrict__ segf,

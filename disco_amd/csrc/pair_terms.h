@@ -47,6 +47,16 @@
 #define DISCO_HD inline
 #endif
 
+// DISCO_HOLD("+v"(a), ...) ends the computation of a, ... at this point of
+// the device program: an empty asm statement that reads and writes them, so
+// the compiler can neither defer their arithmetic past later code nor hoist
+// it.  Nothing on the host.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define DISCO_HOLD(...) asm volatile("" : __VA_ARGS__)
+#else
+#define DISCO_HOLD(...) ((void)0)
+#endif
+
 namespace disco {
 
 constexpr double kPi = 3.14159265358979323846;
@@ -500,6 +510,12 @@ DISCO_HD void flat_step(const SegP& g, float q1, float q2, float h2, float ah, f
         w.x2 = (w.x2 + al * w.a2) + be * g.t2;
     }
     w.sal += al;
+    // the step's sums end here: deferred past the next step's branches (as
+    // the compiler otherwise does) they keep this step's A - p alive beside
+    // the next one's, which costs register copies at the loop's back edge
+    if (KIND == LOOP_GLIDE) DISCO_HOLD("+v"(w.X), "+v"(w.Y), "+v"(w.x1), "+v"(w.x2), "+v"(w.sw3), "+v"(w.sal));
+    else if (KIND == LOOP_PRISM) DISCO_HOLD("+v"(w.X), "+v"(w.Y), "+v"(w.x1), "+v"(w.x2), "+v"(w.sw1), "+v"(w.sw2), "+v"(w.sal));
+    else DISCO_HOLD("+v"(w.X), "+v"(w.Y), "+v"(w.x1), "+v"(w.x2), "+v"(w.sw1), "+v"(w.sw2), "+v"(w.sw3), "+v"(w.sal));
 
     w.a1 = p1; w.a2 = p2;
     w.la2 = lb2; w.RA = RB; w.rA = rB;
@@ -864,6 +880,9 @@ DISCO_HD void flat_step_d(const SegPD& g, double q1, double q2, double h2, doubl
         w.x2 = (w.x2 + al * w.a2) + be * g.t2;
     }
     w.sal += al;
+    // the step's sums end here (as in flat_step)
+    if (GLIDE) DISCO_HOLD("+v"(w.X), "+v"(w.Y), "+v"(w.x1), "+v"(w.x2), "+v"(w.sw3), "+v"(w.sal));
+    else DISCO_HOLD("+v"(w.X), "+v"(w.Y), "+v"(w.x1), "+v"(w.x2), "+v"(w.sw1), "+v"(w.sw2), "+v"(w.sw3), "+v"(w.sal));
 
     w.a1 = p1; w.a2 = p2;
     w.la2 = lb2; w.RA = RB; w.rA = rB;
