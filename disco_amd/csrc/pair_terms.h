@@ -804,21 +804,24 @@ DISCO_HD double two_atanh_vote_d(double z)
 {
     const double y = z * z;
     // z < 1/16: the series to z^13 (the first omitted term, 2 z^15 / 15, is
-    // below 2^-59 of the sum); the one to z^27 for z < 1/4 only when a lane of
-    // the wave needs it, so each lane's result depends on its own z alone
+    // below 2^-59 of the sum; its coefficients are the long series' own, which
+    // keeps the constants of the two in the same registers: an economised
+    // degree-4 form with one multiply-add fewer measured 4.6 % slower); the one
+    // to z^27 for z < 1/4 only when a lane of the wave needs it, so each lane's
+    // result depends on its own z alone
     double p = 1.0 / 13;
     p = p * y + 1.0 / 11; p = p * y + 1.0 / 9;
     p = p * y + 1.0 / 7;  p = p * y + 1.0 / 5;  p = p * y + 1.0 / 3;
     const bool mid = !(z < 0.0625);
-    if (__builtin_expect(M::any(mid), 0)) {
-        double q = 1.0 / 27;
-        q = q * y + 1.0 / 25; q = q * y + 1.0 / 23; q = q * y + 1.0 / 21;
-        q = q * y + 1.0 / 19; q = q * y + 1.0 / 17; q = q * y + 1.0 / 15;
-        q = q * y + 1.0 / 13; q = q * y + 1.0 / 11; q = q * y + 1.0 / 9;
-        q = q * y + 1.0 / 7;  q = q * y + 1.0 / 5;  q = q * y + 1.0 / 3;
-        p = mid ? q : p;
-    }
+    if (__builtin_expect(!M::any(mid), 1)) return 2.0 * z + (2.0 * z * y) * p;
+    double q = 1.0 / 27;
+    q = q * y + 1.0 / 25; q = q * y + 1.0 / 23; q = q * y + 1.0 / 21;
+    q = q * y + 1.0 / 19; q = q * y + 1.0 / 17; q = q * y + 1.0 / 15;
+    q = q * y + 1.0 / 13; q = q * y + 1.0 / 11; q = q * y + 1.0 / 9;
+    q = q * y + 1.0 / 7;  q = q * y + 1.0 / 5;  q = q * y + 1.0 / 3;
+    p = mid ? q : p;
     double r = 2.0 * z + (2.0 * z * y) * p;
+    // z >= 1/4 implies z >= 1/16: tested only in a wave that has a mid lane
     const bool big = !(z < 0.25);
     if (__builtin_expect(M::any(big), 0)) {                    // a branch, not selects
         const double l = log_ge1_d<M>((1.0 + z) * M::rcp_d(1.0 - z));

@@ -1,5 +1,5 @@
-"""The elementary functions of the kernels (FP64 log for w >= 1 and atan2,
-FP32 atan2, in disco_amd/csrc/pair_terms.h), compiled for the host with the
+"""The elementary functions of the kernels (FP64 log for w >= 1, atan2 and the
+walks' 2 atanh, FP32 atan2, in disco_amd/csrc/pair_terms.h), compiled for the host with the
 same source and checked against the C library over wide argument ranges: at
 most a few units in the last place.  (The device build runs the same code, with
 the hardware reciprocal in place of the host's division; the parity tests check
@@ -33,6 +33,7 @@ struct HostMath {
     static bool signbit_d(double x) { return std::signbit(x); }
     static int exponent_d(double x) { int e; std::frexp(x, &e); return e; }
     static double ldexp_d(double x, int e) { return std::ldexp(x, e); }
+    static bool any(bool p) { return p; }      // one lane: each z takes its own branch
 };
 
 static double ulps(double got, double want)
@@ -83,7 +84,20 @@ int main()
             if (x == 0.0f && y == 0.0f) { if (a != 0.0f || std::signbit(a) != std::signbit(y)) bad_signs++; continue; }
             if (std::signbit(a) != std::signbit(b) || std::fabs(a - b) > 4e-7 * std::fmax(std::fabs(b), 1e-30)) bad_signs++;
         }
-    std::printf("%.3f %.3f %ld %.3f\n", worst_log, worst_atan, bad_signs, worst_f);
+    // 2 atanh(z) of the FP64 walks over its three branches (the series to z^13
+    // below 1/16, to z^27 below 1/4, the log form above), against the C library
+    // in long double
+    double worst_at = 0;
+    for (int i = 0; i < 2000000; i++) {
+        const double z = (i & 1) ? 0.0625 * (0.5 + 0.5 * U(r))
+                                 : std::ldexp(0.5 + 0.5 * U(r), -(int)(40.0 * std::fabs(U(r))));
+        const double zz = (i % 3 == 0) ? 0.999 * (0.5 + 0.5 * U(r)) : z;
+        const long double want = 2.0L * std::atanh((long double)zz);
+        const double got = disco::two_atanh_vote_d<HostMath>(zz);
+        worst_at = std::fmax(worst_at, (double)(std::fabs((long double)got - want) /
+                                                 std::ldexp(1.0L, std::ilogb((double)want) - 52)));
+    }
+    std::printf("%.3f %.3f %ld %.3f %.3f\n", worst_log, worst_atan, bad_signs, worst_f, worst_at);
     return 0;
 }
 '''
@@ -100,7 +114,8 @@ def test_fp64_log_and_atan2_match_the_c_library(tmp_path):
                     str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
     worst_log, worst_atan, bad = float(out[0]), float(out[1]), int(out[2])
-    worst_f = float(out[3])
+    worst_f, worst_at = float(out[3]), float(out[4])
+    assert worst_at <= 4.0, worst_at
     assert worst_log <= 4.0, worst_log
     assert worst_atan <= 4.0, worst_atan
     assert worst_f <= 3.0, worst_f
