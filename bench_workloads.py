@@ -216,7 +216,7 @@ def run_c5(args):
     5 radii for the hexagons, which the two-ring cubature does not represent),
     the two-ring cubature from 3 to 5 radii of the circles, the 9-point cubature
     of the cut beyond, the +-1 images far from every atom
-    on an L/32 grid and the far shell (images 2..4) on an L/16 grid (6-node
+    on an L/32 grid and the far shell (images 2..4) on an L/12 grid (6-node
     interpolation stencils).
     Error against the FP64 plain sum on every 8th atom (every atom with
     --c5-all); the plain GPU sum timed on a contiguous eighth of the atoms and
@@ -268,7 +268,8 @@ def run_c5(args):
     t64 = time.perf_counter() - t0
     p64.free()
     err = max(float((a[idx] - b).abs().max()) for a, b in zip(u, v))
-    print(f"[c5] max|du| vs the FP64 plain sum on {idx.numel()} atoms (every {stride}th): "
+    which = "all atoms" if stride == 1 else f"every {stride}th"
+    print(f"[c5] max|du| vs the FP64 plain sum on {idx.numel()} atoms ({which}): "
           f"{err:.3e} A ({t64:.1f} s of FP64)", flush=True)
     del v
     # the plain GPU sum (FP32C) on a contiguous eighth, scaled to the sample

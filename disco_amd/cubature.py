@@ -171,7 +171,7 @@ class CubatureSum:
     L/32) interpolates their field to within near_tol (far_images) are taken
     out of the line plan and summed as cubatures at that grid's nodes (None:
     every +-1 image stays in the line plan).  The far shell (1, img] is summed
-    as cubatures on a grid of spacing grid_h (default L/16).  Both grids are
+    as cubatures on a grid of spacing grid_h (default L/12).  Both grids are
     interpolated by 6-node (quintic) stencils (order = 4: cubic, with defaults
     L/64 and L/24).  Defaults: measured on the 4M-atom samples at images +-2
     and +-4 (max |du| 1.0e-6 and 2.0e-6 A against the FP64 plain sum).
@@ -196,7 +196,7 @@ class CubatureSum:
         self.grids = []
         self.skipped = 0
         self.skipped_band = 0
-        hf = float(grid_h) if grid_h else min(box) / (16.0 if order == 6 else 24.0)
+        hf = float(grid_h) if grid_h else min(box) / (12.0 if order == 6 else 24.0)
         parts = {}                 # grid spacing -> [(loopon loops, img, inner)]
         base = None
         if near_tol is not None and self.img >= 1:
