@@ -174,7 +174,7 @@ class CubatureSum:
     as cubatures on a grid of spacing grid_h (default L/12).  Both grids are
     interpolated by 6-node (quintic) stencils (order = 4: cubic, with defaults
     L/64 and L/24).  Defaults: measured on the 4M-atom samples at images +-2
-    and +-4 (max |du| 1.0e-6 and 2.0e-6 A against the FP64 plain sum).
+    and +-4 (max |du| 1.0e-6 and 2.1e-6 A against the FP64 plain sum, every atom).
 
     k_band (None: off): lanes between k_band and k_exact cut radii of a loop
     image take the two-ring cubature (centre and two rings of twelve, error
