@@ -477,8 +477,11 @@ __global__ __launch_bounds__(kBlock, (PASS == PASS_FP64 && KIND == FP64_FRAME) ?
     ux[i] = sx; uy[i] = sy; uz[i] = sz;
 }
 
-// FP32C near pass: adds the FP64 field of the near (loop, image) pairs.
-__global__ __launch_bounds__(kBlock) void line_near_kernel(
+// FP32C near pass: adds the FP64 field of the near (loop, image) pairs.  At
+// least 3 waves per SIMD (168 registers; a few values of the set-up in
+// scratch): 3.0 ms against 4.0 at the 2 waves its 174 registers give, on the
+// 200-loop +-4 sample's near slabs, the same bits.
+__global__ __launch_bounds__(kBlock, 3) void line_near_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
     const LineLoop* __restrict__ loops, int nloops, const SegD* __restrict__ segd, double Lx,
@@ -696,9 +699,10 @@ __global__ void reset_counters_kernel(unsigned* __restrict__ c, int n)
 
 // Persistent batched line pass: each workgroup takes items from the queue
 // (one vector atomic per item) until every item is taken; items are sorted by
-// decreasing cost on the host (longest first), so the tail is short.
+// decreasing cost on the host (longest first), so the tail is short.  The
+// near pass at least 3 waves per SIMD, as line_near_kernel.
 template <int BLOCK, int PASS, int KIND>
-__global__ __launch_bounds__(BLOCK) void batch_line_kernel(
+__global__ __launch_bounds__(BLOCK, PASS == PASS_NEAR ? 3 : 1) void batch_line_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz,
     const BatchSample* __restrict__ samples, const BatchItem* __restrict__ items, int nitems,
