@@ -161,6 +161,8 @@ def lib():
         L.disco_hip_plan_set_band.argtypes = [_vp, _d, ctypes.POINTER(_i)]
         L.disco_hip_line_band.argtypes = [ctypes.POINTER(_LineLoop), _i, _dp]
         L.disco_hip_plan_set_image_skip.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint), _i]
+        L.disco_hip_plan_set_image_skip_octants.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint), _i,
+                                                            ctypes.POINTER(ctypes.c_double)]
         L.disco_hip_plan_profile.argtypes = [_vp, _i]
         L.disco_hip_plan_kernel_ms.argtypes = [_vp, _dp, _i]
         L.disco_hip_plan_evals_per_atom.argtypes = [_vp]
@@ -438,6 +440,24 @@ class Plan:
         n = _i(0)
         self.ctx.check(lib().disco_hip_plan_set_band(self._h, float(k_band), ctypes.byref(n)))
         return n.value
+
+    def set_image_skip(self, masks=None, split=None):
+        """Leave images of the +-1 shell out of the sums (the caller sums them
+        another way): masks[l] (bit 9 (i + 1) + 3 (j + 1) + (k + 1)) for every
+        atom, or masks[l, o] for the atoms of octant o = [x >= split[0]] +
+        2 [y >= split[1]] + 4 [z >= split[2]] when split is given; None clears."""
+        if masks is None:
+            self.ctx.check(lib().disco_hip_plan_set_image_skip(self._h, None, 0))
+            return
+        m = np.ascontiguousarray(masks, dtype=np.uint32)
+        ptr = m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint))
+        if split is None:
+            self.ctx.check(lib().disco_hip_plan_set_image_skip(self._h, ptr, m.shape[0]))
+            return
+        if m.ndim != 2 or m.shape[1] != 8:
+            raise ValueError("octant masks must have shape (loops, 8)")
+        sp = (ctypes.c_double * 3)(*[float(v) for v in split])
+        self.ctx.check(lib().disco_hip_plan_set_image_skip_octants(self._h, ptr, m.shape[0], sp))
 
     def profile(self, on: bool = True):
         """Record HIP events around each kernel of later runs."""

@@ -140,8 +140,8 @@ __device__ __forceinline__ bool frame_image(const FrameAtom& f, int di, int dj, 
 // FP64 field of one image of a loop flat to double resolution, evaluated in
 // the loop frame and added in global axes: b fo + o1 e1 + o2 e2 + o3 n.
 __device__ __forceinline__ void add_flat_f64(double q1, double q2, double q3, const LineLoop& L,
-                                             const SegD* __restrict__ segd, double& sx,
-                                             double& sy, double& sz)
+                                             const SegD* __restrict__ segd, bool skip,
+                                             double& sx, double& sy, double& sz)
 {
     double fo, o1, o2, o3;
     const SegPD* sp = reinterpret_cast<const SegPD*>(segd + L.seg0);
@@ -149,6 +149,7 @@ __device__ __forceinline__ void add_flat_f64(double q1, double q2, double q3, co
         line_image_flat_f64<DevMath, true>(q1, q2, q3, sp, L.nv, fo, o1, o2, o3);
     else
         line_image_flat_f64<DevMath, false>(q1, q2, q3, sp, L.nv, fo, o1, o2, o3);
+    if (skip) return;                              // a lane that leaves the image out
     sx += L.bx * fo + ((o1 * L.e1x + o2 * L.e2x) + o3 * L.nx);
     sy += L.by * fo + ((o1 * L.e1y + o2 * L.e2y) + o3 * L.ny);
     sz += L.bz * fo + ((o1 * L.e1z + o2 * L.e2z) + o3 * L.nz);
@@ -176,13 +177,50 @@ enum { PASS_FP64 = 0, PASS_FAR = 1, PASS_NEAR = 2 };
 // fewer registers.
 enum { FP64_GLOBAL = 1, FP64_FRAME = 2 };
 
-// Images a line pass leaves out: the inner shell (max(|i|,|j|,|k|) <= inner)
-// and the loop's skipped images of [-1, 1]^3 (wave-uniform).
-__device__ __forceinline__ bool image_out(const LineLoop& L, int di, int dj, int dk, int inner)
+// Images a line pass leaves out.  The inner shell (max(|i|,|j|,|k|) <= inner)
+// is left out for every lane; an image of [-1, 1]^3 is left out for the lanes
+// whose skip mask holds its bit (skip_bit): the mask of the lane's octant
+// (lane_skip).  A wave passes over an image only when every lane leaves it out;
+// otherwise the lanes that leave it out evaluate it with the others and drop
+// the result.
+__device__ __forceinline__ bool inner_out(int di, int dj, int dk, int inner)
 {
-    const int m = max(abs(di), max(abs(dj), abs(dk)));
-    if (m <= inner) return true;
-    return m <= 1 && ((L.img_skip >> ((di + 1) * 9 + (dj + 1) * 3 + (dk + 1))) & 1u);
+    return max(abs(di), max(abs(dj), abs(dk))) <= inner;
+}
+
+__device__ __forceinline__ bool skip_bit(unsigned mask, int di, int dj, int dk)
+{
+    return max(abs(di), max(abs(dj), abs(dk))) <= 1 &&
+           ((mask >> ((di + 1) * 9 + (dj + 1) * 3 + (dk + 1))) & 1u);
+}
+
+// Whether the lane leaves image (di, dj, dk) of loop L out (lm: its lane_skip
+// mask); a wave-uniform branch keeps loops that leave nothing out free of it.
+__device__ __forceinline__ bool lane_skips_image(const LineLoop& L, unsigned lm, int di, int dj,
+                                                 int dk)
+{
+    return L.skip_any ? skip_bit(lm, di, dj, dk) : false;
+}
+
+// The octant of the atom for the per-octant skip masks (all loops of a plan
+// hold the same planes osx, osy, osz).
+__device__ __forceinline__ int atom_octant(double px, double py, double pz, const LineLoop& L)
+{
+    return (px >= L.osx ? 1 : 0) | (py >= L.osy ? 2 : 0) | (pz >= L.osz ? 4 : 0);
+}
+
+// The lane's skip mask of loop L: oct_skip[oct] selected from the eight
+// wave-uniform masks (scalar loads; no per-lane memory access); 0 when the
+// loop leaves nothing out.
+__device__ __forceinline__ unsigned lane_skip(const LineLoop& L, int oct)
+{
+    if (!L.skip_any) return 0u;
+    const unsigned a = (oct & 1) ? L.oct_skip[1] : L.oct_skip[0];
+    const unsigned b = (oct & 1) ? L.oct_skip[3] : L.oct_skip[2];
+    const unsigned c = (oct & 1) ? L.oct_skip[5] : L.oct_skip[4];
+    const unsigned d = (oct & 1) ? L.oct_skip[7] : L.oct_skip[6];
+    const unsigned e = (oct & 2) ? b : a, g = (oct & 2) ? d : c;
+    return (oct & 4) ? g : e;
 }
 
 // FP32 far pass of one flat loop: every lane evaluates every image (uniform
@@ -195,74 +233,110 @@ __device__ __forceinline__ bool image_out(const LineLoop& L, int di, int dj, int
 // and the cubature only when one has w > 0.  With the band form on
 // (disco_hip_plan_set_band), the two-ring cubature takes the exact field's
 // place for lanes with |q|^2 >= rtb.
-template <int KIND, bool CUBATURE>
+// One image (di, dj, dk) of a flat loop in the FP32 far pass (SKIPS: the plan
+// leaves images out for some atoms).
+template <int KIND, bool CUBATURE, bool SKIPS>
+__device__ __forceinline__ void far_flat_image(const FrameAtom& f, const LineLoop& L,
+                                               const SegP* __restrict__ s,
+                                               const SegD* __restrict__ segd, int img, int di,
+                                               int dj, int dk, unsigned lm, double& sx,
+                                               double& sy, double& sz)
+{
+    const bool skip = SKIPS ? lane_skips_image(L, lm, di, dj, dk) : false;
+    if (SKIPS && L.skip_any && DevMath::all(skip)) return;
+    double q1, q2, q3;
+    // a lane that leaves the image out counts as near: the far pass
+    // drops its result (the near pass never sees such an image)
+    const bool near = frame_image(f, di, dj, dk, img, L, segd, q1, q2, q3) || skip;
+    const float g1 = (float)q1, g2 = (float)q2, g3 = (float)q3;
+    float fo = 0.f, o1 = 0.f, o2 = 0.f, o3 = 0.f;
+    if (!CUBATURE || !L.rule_ok) {
+        line_image_flat_f32<DevMath, KIND>(g1, g2, g3, s, L.nv, fo, o1, o2, o3);
+    } else {
+        const float t = g1 * g1 + (g2 * g2 + g3 * g3);
+        // a hard switch at rt0 (k_rule = k_exact, rwinv huge): the
+        // window is the step that rule_window gives there, and no
+        // lane blends
+        const bool hard = L.rwinv >= 1e29f;
+        const float w = hard ? (t > L.rt0 ? 1.0f : 0.0f) : rule_window(t, L);
+        // the two-ring rule stands in for the exact field beyond rtb
+        // (never inside the near slab; rtb is huge when it is off)
+        const bool band = t >= L.rtb;
+        if (DevMath::any(!near && w < 1.0f && !band))
+            line_image_flat_f32<DevMath, KIND>(g1, g2, g3, s, L.nv, fo, o1, o2, o3);
+        if (DevMath::any(!near && w < 1.0f && band)) {
+            float b1, b2, b3;
+            band_image_f32<DevMath>(g1, g2, g3, t, L, b1, b2, b3);
+            fo = band ? 0.0f : fo;
+            o1 = band ? b1 : o1;
+            o2 = band ? b2 : o2;
+            o3 = band ? b3 : o3;
+        }
+        if (DevMath::any(!near && w > 0.0f)) {
+            float c1, c2, c3;
+            rule9_image_f32<DevMath>(g1, g2, g3, t, L, c1, c2, c3);
+            // selects, so that a field not evaluated (or not finite
+            // where its weight is zero) never enters the sum
+            const bool all_c = w >= 1.0f;
+            if (hard) {
+                fo = all_c ? 0.0f : fo;
+                o1 = all_c ? c1 : o1;
+                o2 = all_c ? c2 : o2;
+                o3 = all_c ? c3 : o3;
+            } else {
+                const float we = 1.0f - w;
+                const bool mix = w > 0.0f && !all_c;
+                fo = all_c ? 0.0f : (mix ? we * fo : fo);
+                o1 = all_c ? c1 : (mix ? we * o1 + w * c1 : o1);
+                o2 = all_c ? c2 : (mix ? we * o2 + w * c2 : o2);
+                o3 = all_c ? c3 : (mix ? we * o3 + w * c3 : o3);
+            }
+        }
+    }
+    if (!near) {
+        sx += (double)(L.bf[0] * fo + (o1 * L.e1f[0] + o2 * L.e2f[0] + o3 * L.nf[0]));
+        sy += (double)(L.bf[1] * fo + (o1 * L.e1f[1] + o2 * L.e2f[1] + o3 * L.nf[1]));
+        sz += (double)(L.bf[2] * fo + (o1 * L.e1f[2] + o2 * L.e2f[2] + o3 * L.nf[2]));
+    }
+}
+
+// The images of a flat loop in the FP32 far pass, in the order of (di, dj, dk)
+// with dk fastest.  A +-1 plan whose loop leaves images out (wskip: the images
+// every lane of the wave leaves out) visits only the others, in the same order
+// (bit 9 (di + 1) + 3 (dj + 1) + (dk + 1) rises with it).
+template <int KIND, bool CUBATURE, bool SKIPS>
 __device__ __forceinline__ void far_flat_loop(const FrameAtom& f, const LineLoop& L,
                                               const SegP* __restrict__ s,
                                               const SegD* __restrict__ segd, int img, int inner,
-                                              double& sx, double& sy, double& sz)
+                                              unsigned lm, unsigned wskip, double& sx,
+                                              double& sy, double& sz)
 {
+    if (SKIPS && img == 1 && L.skip_any) {
+        unsigned bits = ((1u << 27) - 1u) & ~wskip;
+        if (inner >= 1) bits = 0u;
+        else if (inner == 0) bits &= ~(1u << 13);          // the central image
+        while (bits) {
+            const int b = __builtin_ctz(bits);
+            bits &= bits - 1u;
+            far_flat_image<KIND, CUBATURE, SKIPS>(f, L, s, segd, 1, b / 9 - 1, (b / 3) % 3 - 1,
+                                                  b % 3 - 1, lm, sx, sy, sz);
+        }
+        return;
+    }
     for (int di = -img; di <= img; di++)
         for (int dj = -img; dj <= img; dj++)
             for (int dk = -img; dk <= img; dk++) {
-                if (image_out(L, di, dj, dk, inner)) continue;
-                double q1, q2, q3;
-                const bool near = frame_image(f, di, dj, dk, img, L, segd, q1, q2, q3);
-                const float g1 = (float)q1, g2 = (float)q2, g3 = (float)q3;
-                float fo = 0.f, o1 = 0.f, o2 = 0.f, o3 = 0.f;
-                if (!CUBATURE || !L.rule_ok) {
-                    line_image_flat_f32<DevMath, KIND>(g1, g2, g3, s, L.nv, fo, o1, o2, o3);
-                } else {
-                    const float t = g1 * g1 + (g2 * g2 + g3 * g3);
-                    // a hard switch at rt0 (k_rule = k_exact, rwinv huge): the
-                    // window is the step that rule_window gives there, and no
-                    // lane blends
-                    const bool hard = L.rwinv >= 1e29f;
-                    const float w = hard ? (t > L.rt0 ? 1.0f : 0.0f) : rule_window(t, L);
-                    // the two-ring rule stands in for the exact field beyond rtb
-                    // (never inside the near slab; rtb is huge when it is off)
-                    const bool band = t >= L.rtb;
-                    if (DevMath::any(!near && w < 1.0f && !band))
-                        line_image_flat_f32<DevMath, KIND>(g1, g2, g3, s, L.nv, fo, o1, o2, o3);
-                    if (DevMath::any(w < 1.0f && band)) {
-                        float b1, b2, b3;
-                        band_image_f32<DevMath>(g1, g2, g3, t, L, b1, b2, b3);
-                        fo = band ? 0.0f : fo;
-                        o1 = band ? b1 : o1;
-                        o2 = band ? b2 : o2;
-                        o3 = band ? b3 : o3;
-                    }
-                    if (DevMath::any(w > 0.0f)) {
-                        float c1, c2, c3;
-                        rule9_image_f32<DevMath>(g1, g2, g3, t, L, c1, c2, c3);
-                        // selects, so that a field not evaluated (or not finite
-                        // where its weight is zero) never enters the sum
-                        const bool all_c = w >= 1.0f;
-                        if (hard) {
-                            fo = all_c ? 0.0f : fo;
-                            o1 = all_c ? c1 : o1;
-                            o2 = all_c ? c2 : o2;
-                            o3 = all_c ? c3 : o3;
-                        } else {
-                            const float we = 1.0f - w;
-                            const bool mix = w > 0.0f && !all_c;
-                            fo = all_c ? 0.0f : (mix ? we * fo : fo);
-                            o1 = all_c ? c1 : (mix ? we * o1 + w * c1 : o1);
-                            o2 = all_c ? c2 : (mix ? we * o2 + w * c2 : o2);
-                            o3 = all_c ? c3 : (mix ? we * o3 + w * c3 : o3);
-                        }
-                    }
-                }
-                if (!near) {
-                    sx += (double)(L.bf[0] * fo + (o1 * L.e1f[0] + o2 * L.e2f[0] + o3 * L.nf[0]));
-                    sy += (double)(L.bf[1] * fo + (o1 * L.e1f[1] + o2 * L.e2f[1] + o3 * L.nf[1]));
-                    sz += (double)(L.bf[2] * fo + (o1 * L.e1f[2] + o2 * L.e2f[2] + o3 * L.nf[2]));
-                }
+                if (inner_out(di, dj, dk, inner)) continue;
+                far_flat_image<KIND, CUBATURE, SKIPS>(f, L, s, segd, img, di, dj, dk, lm, sx, sy,
+                                                      sz);
             }
 }
 
 // PASS_FAR sums loops of one kind (KIND), each kind in its own kernel so that
-// each is compiled (and its registers allocated) for its own form.
-template <int PASS, int KIND = LOOP_3D, bool CUBATURE = false>
+// each is compiled (and its registers allocated) for its own form.  SKIPS: the
+// plan leaves images out for some atoms (image skips); the kernels of plans
+// that do not carry none of that work.
+template <int PASS, int KIND = LOOP_3D, bool CUBATURE = false, bool SKIPS = false>
 __device__ __forceinline__ bool atom_line(double px, double py, double pz,
                                           const LineLoop* __restrict__ loops, int l0, int l1,
                                           const SegF* __restrict__ segf,
@@ -271,6 +345,11 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
                                           double& sz)
 {
     bool any = false;
+    const int oct = SKIPS && l0 < l1 ? atom_octant(px, py, pz, loops[l0]) : 0;
+    unsigned present = 0u;                 // the octants of the wave's lanes
+    if (SKIPS && PASS == PASS_FAR)
+        for (int o = 0; o < 8; o++)
+            if (DevMath::any(oct == o)) present |= 1u << o;
     for (int l = l0; l < l1; l++) {
         const LineLoop& L = loops[l];
         const double pcx = px - L.cx, pcy = py - L.cy, pcz = pz - L.cz;
@@ -278,28 +357,39 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
         if (PASS == PASS_FP64) {
             if ((KIND == FP64_GLOBAL && L.flat64) || (KIND == FP64_FRAME && !L.flat64)) continue;
             if (L.flat64) frame_atom(pcx, pcy, pcz, L, f);
+            const unsigned lm = SKIPS ? lane_skip(L, oct) : 0u;
             for (int di = -img; di <= img; di++)
                 for (int dj = -img; dj <= img; dj++)
                     for (int dk = -img; dk <= img; dk++) {
-                        if (image_out(L, di, dj, dk, inner)) continue;
+                        if (inner_out(di, dj, dk, inner)) continue;
+                        const bool skip = SKIPS ? lane_skips_image(L, lm, di, dj, dk) : false;
+                        if (SKIPS && L.skip_any && DevMath::all(skip)) continue;
                         if (L.flat64) {
                             double q1, q2, q3;
                             frame_image(f, di, dj, dk, img, L, segd, q1, q2, q3);
-                            add_flat_f64(q1, q2, q3, L, segd, sx, sy, sz);
+                            add_flat_f64(q1, q2, q3, L, segd, skip, sx, sy, sz);
                             continue;
                         }
                         const double qx = pcx - di * Lx, qy = pcy - dj * Ly, qz = pcz - dk * Lz;
                         double ox = 0.0, oy = 0.0, oz = 0.0;
                         line_image_f64<DevMath>(qx, qy, qz, segd + L.seg0, L.nv, L.bx, L.by,
                                                 L.bz, ox, oy, oz);
-                        sx += ox; sy += oy; sz += oz;
+                        if (!skip) { sx += ox; sy += oy; sz += oz; }
                     }
             continue;
         }
         frame_atom(pcx, pcy, pcz, L, f);
         if (PASS == PASS_FAR && KIND != LOOP_3D) {
             const SegP* sp = reinterpret_cast<const SegP*>(segf + L.seg0);
-            far_flat_loop<KIND, CUBATURE>(f, L, sp, segd, img, inner, sx, sy, sz);
+            unsigned wskip = 0u;           // images every lane of the wave leaves out
+            if (SKIPS && L.skip_any) {
+                wskip = ~0u;
+                for (int o = 0; o < 8; o++)
+                    if ((present >> o) & 1u) wskip &= L.oct_skip[o];
+            }
+            far_flat_loop<KIND, CUBATURE, SKIPS>(f, L, sp, segd, img, inner,
+                                                 SKIPS ? lane_skip(L, oct) : 0u, wskip, sx, sy,
+                                                 sz);
             continue;
         }
         if (PASS == PASS_NEAR) {
@@ -311,12 +401,17 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
             for (int di = i0; di <= i1; di++)
                 for (int dj = j0; dj <= j1; dj++)
                     for (int dk = k0; dk <= k1; dk++) {
-                        if (image_out(L, di, dj, dk, inner)) continue;
+                        // a lane near an image never leaves it out (a left-out
+                        // image lies beyond the near slab of every lane that
+                        // leaves it out): only images left out for every lane
+                        // are passed over
+                        if (inner_out(di, dj, dk, inner) || skip_bit(L.skip_all, di, dj, dk))
+                            continue;
                         double q1, q2, q3;
                         if (!frame_image(f, di, dj, dk, img, L, segd, q1, q2, q3)) continue;
                         any = true;
                         if (L.flat64) {
-                            add_flat_f64(q1, q2, q3, L, segd, sx, sy, sz);
+                            add_flat_f64(q1, q2, q3, L, segd, false, sx, sy, sz);
                             continue;
                         }
                         const double qx = pcx - di * Lx, qy = pcy - dj * Ly, qz = pcz - dk * Lz;
@@ -327,28 +422,22 @@ __device__ __forceinline__ bool atom_line(double px, double py, double pz,
                     }
             continue;
         }
+        // PASS_FAR, LOOP_3D: the global-axes FP32 form
+        const unsigned lm = SKIPS ? lane_skip(L, oct) : 0u;
         for (int di = -img; di <= img; di++)
             for (int dj = -img; dj <= img; dj++)
                 for (int dk = -img; dk <= img; dk++) {
-                    if (image_out(L, di, dj, dk, inner)) continue;
+                    if (inner_out(di, dj, dk, inner)) continue;
+                    const bool skip = SKIPS ? lane_skips_image(L, lm, di, dj, dk) : false;
+                    if (SKIPS && L.skip_any && DevMath::all(skip)) continue;
                     double q1, q2, q3;
                     const bool near = frame_image(f, di, dj, dk, img, L, segd, q1, q2, q3);
                     const double qx = pcx - di * Lx, qy = pcy - dj * Ly, qz = pcz - dk * Lz;
-                    if (PASS == PASS_FAR) {
-                        // LOOP_3D: the global-axes FP32 form
-                        float ox = 0.f, oy = 0.f, oz = 0.f;
-                        line_image_f32<DevMath>((float)qx, (float)qy, (float)qz, segf + L.seg0,
-                                                L.nv, (float)L.bx, (float)L.by, (float)L.bz,
-                                                ox, oy, oz);
-                        if (!near) {
-                            sx += (double)ox; sy += (double)oy; sz += (double)oz;
-                        }
-                    } else if (near) {
-                        double ox = 0.0, oy = 0.0, oz = 0.0;
-                        line_image_f64<DevMath>(qx, qy, qz, segd + L.seg0, L.nv, L.bx, L.by,
-                                                L.bz, ox, oy, oz);
-                        sx += ox; sy += oy; sz += oz;
-                        any = true;
+                    float ox = 0.f, oy = 0.f, oz = 0.f;
+                    line_image_f32<DevMath>((float)qx, (float)qy, (float)qz, segf + L.seg0, L.nv,
+                                            (float)L.bx, (float)L.by, (float)L.bz, ox, oy, oz);
+                    if (!near && !skip) {
+                        sx += (double)ox; sy += (double)oy; sz += (double)oz;
                     }
                 }
     }
@@ -460,7 +549,7 @@ __device__ __forceinline__ void atom_loopon(double px, double py, double pz,
 // none of the walk's; 1,490 ms against 1,510 at 4 waves and 1,561 at 3 on the
 // 4M-atom sample, the same bits); the global-axes FP64 walk keeps 3 (168
 // registers, no scratch); the FP32 ones use at most 74 registers either way.
-template <int PASS, int KIND, bool CUBATURE = false>
+template <int PASS, int KIND, bool CUBATURE = false, bool SKIPS = false>
 __global__ __launch_bounds__(kBlock, (PASS == PASS_FP64 && KIND == FP64_FRAME) ? 5 : 3) void line_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
@@ -471,7 +560,7 @@ __global__ __launch_bounds__(kBlock, (PASS == PASS_FP64 && KIND == FP64_FRAME) ?
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    atom_line<PASS, KIND, CUBATURE>(x[i], y[i], z[i], loops, 0, nloops, segf, segd, Lx, Ly, Lz,
+    atom_line<PASS, KIND, CUBATURE, SKIPS>(x[i], y[i], z[i], loops, 0, nloops, segf, segd, Lx, Ly, Lz,
                                     img, inner, sx, sy, sz);
     if (accumulate) { sx += ux[i]; sy += uy[i]; sz += uz[i]; }
     ux[i] = sx; uy[i] = sy; uz[i] = sz;
@@ -778,6 +867,7 @@ struct disco_plan {
     double L[3] = {0, 0, 0};
     double work = 0;       // segments (or loopons) per image
     int kofs[5] = {0, 0, 0, 0, 0};   // line plans: loops of kind k are [kofs[k], kofs[k+1])
+    bool skips = false;              // line plans: some loop leaves images out (image skips)
     bool cubature = false;           // FP32C line plans: the far pass weighs in the cubature
     std::vector<LineLoop> hloops;    // line plans: host copy of the packed loops
     std::vector<int> src;            // input index of each packed loop
@@ -866,6 +956,27 @@ void free_plan(disco_plan* p)
     delete p;
 }
 
+// The FP32C far-pass kernel of loop kind k (SK: the plan leaves images out).
+using LineKernel = void (*)(const double*, const double*, const double*, double*, double*,
+                            double*, int, const LineLoop*, int, const SegF*, const SegD*, double,
+                            double, double, int, int, int);
+template <bool SK>
+static LineKernel far_kernel(int k, bool cub)
+{
+    switch (k) {
+    case LOOP_3D: return line_kernel<PASS_FAR, LOOP_3D, false, SK>;
+    case LOOP_FLAT:
+        return cub ? line_kernel<PASS_FAR, LOOP_FLAT, true, SK>
+                   : line_kernel<PASS_FAR, LOOP_FLAT, false, SK>;
+    case LOOP_GLIDE:
+        return cub ? line_kernel<PASS_FAR, LOOP_GLIDE, true, SK>
+                   : line_kernel<PASS_FAR, LOOP_GLIDE, false, SK>;
+    default:
+        return cub ? line_kernel<PASS_FAR, LOOP_PRISM, true, SK>
+                   : line_kernel<PASS_FAR, LOOP_PRISM, false, SK>;
+    }
+}
+
 // scratch = nullptr: a split loopon plan keeps its partial sums in its own
 // buffer (allocated, grow-only); otherwise they go to `scratch`, which holds
 // scratch_bytes (a larger need fails before anything is launched).  la, lb:
@@ -904,12 +1015,14 @@ int launch(disco_plan* p, int n, const double* x, const double* y, const double*
             // loops in global axes, then the loop-frame ones (each kernel
             // skips the others); the first writes u (or adds, as asked).
             // Both always run, so a batch sums in the same order.
-            hipLaunchKernelGGL((line_kernel<PASS_FP64, FP64_GLOBAL>), grid, block, 0, s, x, y, z,
-                               ux, uy, uz, n, lp + la, nl, sf, sd, p->L[0], p->L[1], p->L[2],
-                               p->img, p->inner, accumulate);
-            hipLaunchKernelGGL((line_kernel<PASS_FP64, FP64_FRAME>), grid, block, 0, s, x, y, z,
-                               ux, uy, uz, n, lp + la, nl, sf, sd, p->L[0], p->L[1], p->L[2],
-                               p->img, p->inner, 1);
+            auto kg = p->skips ? line_kernel<PASS_FP64, FP64_GLOBAL, false, true>
+                               : line_kernel<PASS_FP64, FP64_GLOBAL>;
+            auto kf = p->skips ? line_kernel<PASS_FP64, FP64_FRAME, false, true>
+                               : line_kernel<PASS_FP64, FP64_FRAME>;
+            hipLaunchKernelGGL(kg, grid, block, 0, s, x, y, z, ux, uy, uz, n, lp + la, nl, sf, sd,
+                               p->L[0], p->L[1], p->L[2], p->img, p->inner, accumulate);
+            hipLaunchKernelGGL(kf, grid, block, 0, s, x, y, z, ux, uy, uz, n, lp + la, nl, sf, sd,
+                               p->L[0], p->L[1], p->L[2], p->img, p->inner, 1);
         } else {
             // far pass: one kernel per loop kind present; the first writes u
             // (or adds, as asked), the others add
@@ -918,14 +1031,8 @@ int launch(disco_plan* p, int n, const double* x, const double* y, const double*
                 const int l0 = std::max(p->kofs[k], la);
                 const int cnt = std::min(p->kofs[k + 1], lb) - l0;
                 if (cnt <= 0) continue;
-                const bool cub = p->cubature;
-                auto kern = k == LOOP_3D ? line_kernel<PASS_FAR, LOOP_3D>
-                          : k == LOOP_FLAT ? (cub ? line_kernel<PASS_FAR, LOOP_FLAT, true>
-                                                  : line_kernel<PASS_FAR, LOOP_FLAT>)
-                          : k == LOOP_GLIDE ? (cub ? line_kernel<PASS_FAR, LOOP_GLIDE, true>
-                                                   : line_kernel<PASS_FAR, LOOP_GLIDE>)
-                          : (cub ? line_kernel<PASS_FAR, LOOP_PRISM, true>
-                                 : line_kernel<PASS_FAR, LOOP_PRISM>);
+                auto kern = p->skips ? far_kernel<true>(k, p->cubature)
+                                     : far_kernel<false>(k, p->cubature);
                 hipLaunchKernelGGL(kern, grid, block, 0, s, x, y, z, ux, uy, uz, n, lp + l0, cnt,
                                    sf, sd, p->L[0], p->L[1], p->L[2], p->img, p->inner, acc);
                 acc = 1;
@@ -1619,7 +1726,10 @@ int disco_hip_plan_set_band(disco_plan* plan, double k_band, int* eligible)
     return DISCO_OK;
 }
 
-int disco_hip_plan_set_image_skip(disco_plan* plan, const unsigned* skip, int nloops)
+// The image skips of a line plan: `per` masks per input loop (1: one for every
+// atom, 8: one per octant of the planes split[3]), or none (skip = NULL).
+static int set_image_skips(disco_plan* plan, const unsigned* skip, int nloops, int per,
+                           const double* split)
 {
     if (!plan) return DISCO_ERR_ARG;
     disco_ctx* ctx = plan->ctx;
@@ -1628,7 +1738,26 @@ int disco_hip_plan_set_image_skip(disco_plan* plan, const unsigned* skip, int nl
     if (skip)
         for (const LineLoop& L : plan->hloops)
             if (L.src >= nloops) return fail(ctx, DISCO_ERR_ARG, "skip array shorter than the loops");
-    for (LineLoop& L : plan->hloops) L.img_skip = skip ? skip[L.src] & ((1u << 27) - 1u) : 0u;
+    if (split)
+        for (int k = 0; k < 3; k++)
+            if (std::isnan(split[k])) return fail(ctx, DISCO_ERR_ARG, "octant plane is NaN");
+    for (LineLoop& L : plan->hloops) {
+        unsigned all = ~0u, any = 0u;
+        for (int o = 0; o < 8; o++) {
+            const unsigned m = skip ? skip[(size_t)L.src * per + (per == 8 ? o : 0)] & ((1u << 27) - 1u)
+                                    : 0u;
+            L.oct_skip[o] = m;
+            all &= m;
+            any |= m;
+        }
+        L.skip_all = all;
+        L.skip_any = any;
+        L.osx = split ? split[0] : INFINITY;
+        L.osy = split ? split[1] : INFINITY;
+        L.osz = split ? split[2] : INFINITY;
+    }
+    plan->skips = false;
+    for (const LineLoop& L : plan->hloops) plan->skips = plan->skips || L.skip_any != 0u;
     DISCO_HIP_TRY(ctx, hipSetDevice(ctx->device));
     if (!plan->hloops.empty()) {
         DISCO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -1637,6 +1766,18 @@ int disco_hip_plan_set_image_skip(disco_plan* plan, const unsigned* skip, int nl
                                      hipMemcpyHostToDevice));
     }
     return DISCO_OK;
+}
+
+int disco_hip_plan_set_image_skip(disco_plan* plan, const unsigned* skip, int nloops)
+{
+    return set_image_skips(plan, skip, nloops, 1, nullptr);
+}
+
+int disco_hip_plan_set_image_skip_octants(disco_plan* plan, const unsigned* skip, int nloops,
+                                          const double* split)
+{
+    if (plan && skip && !split) return fail(plan->ctx, DISCO_ERR_ARG, "octant planes missing");
+    return set_image_skips(plan, skip, nloops, 8, skip ? split : nullptr);
 }
 
 int disco_hip_line_cubature(const disco_line_loop* loops, int nloops, double* out)

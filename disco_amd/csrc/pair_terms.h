@@ -115,8 +115,6 @@ struct alignas(64) LineLoop {
     int rule_ok;            // the cubature matches the cut's moments through degree 5
     float rR;               // the cut's radius: largest in-plane vertex distance from c
     int src;                // index of the loop in the caller's loop array
-    unsigned img_skip;      // images (i, j, k) in [-1, 1]^3 left out of the sum: bit
-                            // 9 (i + 1) + 3 (j + 1) + (k + 1) (the caller sums them)
     // Two-ring cubature of the cut (the band form of a cubature plan): the
     // centre (area bw0) and two rings of twelve points (areas bw1, bw2 each) at
     // radii brho1, brho2, ring 1 at angles j pi/6 from e1, ring 2 at
@@ -128,9 +126,17 @@ struct alignas(64) LineLoop {
     float rtb;
     int band_ok;            // the two-ring rule matches the cut's moments through degree 9
     int sh0;                // first FP64-table slot of the loop's image shifts (plans)
-    int rpad2[7];
+    // Images (i, j, k) of [-1, 1]^3 left out of the sum (the caller sums them
+    // another way): bit 9 (i + 1) + 3 (j + 1) + (k + 1) of oct_skip[o] for the
+    // atoms of octant o = [x >= osx] + 2 [y >= osy] + 4 [z >= osz]; skip_all is
+    // the AND of the eight masks (images left out for every atom), skip_any
+    // their OR (0: nothing left out)
+    double osx, osy, osz;
+    unsigned oct_skip[8];
+    unsigned skip_all, skip_any;
+    int rpad2[8];
 };
-static_assert(sizeof(LineLoop) == 512, "loop record size");
+static_assert(sizeof(LineLoop) == 576, "loop record size");
 
 // One loop of a loopon plan; loopons are stored relative to (cx, cy, cz).
 struct alignas(128) LooponLoop {

@@ -31,7 +31,6 @@ tests compare it with the FP64 plain sum.
 """
 from __future__ import annotations
 
-import ctypes
 import itertools
 
 import numpy as np
@@ -131,11 +130,18 @@ def band_loopons(loops, band):
 
 class _Grid:
     """Loopon plans summed at the nodes of a uniform grid over the atoms and
-    interpolated to them (tensor-product Lagrange on `order` nodes per axis)."""
+    interpolated to them (tensor-product Lagrange on `order` nodes per axis).
+    With idx (a device index tensor), the grid covers the atoms idx of the
+    arrays run() is given, and adds to their u only."""
 
-    def __init__(self, ctx, x, y, z, box, h, parts, groups=None, order=6):
+    def __init__(self, ctx, x, y, z, box, h, parts, groups=None, order=6, idx=None):
         import torch
         self.ctx, self.h, self.order = ctx, float(h), int(order)
+        self.idx = idx
+        if idx is not None:
+            x, y, z = x[idx].contiguous(), y[idx].contiguous(), z[idx].contiguous()
+            self.sub = (x, y, z)
+            self.su = [torch.zeros_like(x) for _ in range(3)]
         self.origin, dims, self.nodes = S.grid_nodes(x, y, z, self.h, margin=self.order // 2 - 1)
         tiles = (self.nodes[0].numel() + 255) // 256
         self.plans = []
@@ -152,8 +158,15 @@ class _Grid:
     def run(self, x, y, z, ux, uy, uz):
         for k, p in enumerate(self.plans):
             p.run_torch(*self.nodes, *self.gu, accumulate=k > 0)
-        B.grid_interp_torch(self.ctx, x, y, z, ux, uy, uz, [t.view(self.shape) for t in self.gu],
-                            self.origin, self.h, accumulate=True, order=self.order)
+        g = [t.view(self.shape) for t in self.gu]
+        if self.idx is None:
+            B.grid_interp_torch(self.ctx, x, y, z, ux, uy, uz, g, self.origin, self.h,
+                                accumulate=True, order=self.order)
+            return
+        B.grid_interp_torch(self.ctx, *self.sub, *self.su, g, self.origin, self.h,
+                            accumulate=False, order=self.order)
+        for u, v in zip((ux, uy, uz), self.su):
+            u.index_add_(0, self.idx, v)
 
     def free(self):
         for p in self.plans:
@@ -182,11 +195,18 @@ class CubatureSum:
     represents (circles and regular polygons of twelve sides or more); the
     +-1 images at least k_band R + 3 near_h from the atoms (and beyond the
     interpolation bound) then go to the near_h grid with that cubature
-    (far_images)."""
+    (far_images).
+
+    octants (default True): the +-1 images are chosen per octant of the atoms
+    (split at the middle of their bounding box): an image far enough from the
+    atoms of one octant goes to that octant's near_h grid and is left out of
+    the line plan for those atoms only (Plan.set_image_skip with split), so
+    far fewer (atom, loop image) pairs stay exact; False: one grid, images far
+    from every atom."""
 
     def __init__(self, ctx, x, y, z, loops, box, img, k_exact=5.0, k_rule=None,
                  grid_h=None, grid_groups=None, near_tol=1e-7, near_h=None, order=6,
-                 k_band=None):
+                 k_band=None, octants=True):
         self.ctx = ctx
         self.img = int(img)
         self.near = B.Plan(ctx, loops, box, min(1, self.img), B.PREC_FP32C)
@@ -199,7 +219,9 @@ class CubatureSum:
         hf = float(grid_h) if grid_h else min(box) / (12.0 if order == 6 else 24.0)
         parts = {}                 # grid spacing -> [(loopon loops, img, inner)]
         base = None
+        near_grids = []            # (atom index or None, parts) of the near_h grids
         if near_tol is not None and self.img >= 1:
+            import torch
             hn = float(near_h) if near_h else min(box) / (32.0 if order == 6 else 64.0)
             lo = [float(t.min()) for t in (x, y, z)]
             hi = [float(t.max()) for t in (x, y, z)]
@@ -211,26 +233,51 @@ class CubatureSum:
             if k_band:
                 band = B.line_band(loops)
                 band[:, 0] *= info[:, 0]        # the grid needs the ring rule's loops
-            masks, far = far_images(loops, box, lo, hi, hn, float(near_tol),
-                                    max(float(k_exact), float(k_rule or 0.0)), info=info,
+            kx = max(float(k_exact), float(k_rule or 0.0))
+            # the atoms of each octant (all atoms in one set without octants);
+            # the kernels take the same comparisons against the same planes
+            split = [0.5 * (a + b) for a, b in zip(lo, hi)]
+            if octants:
+                code = ((x >= split[0]).int() + 2 * (y >= split[1]).int()
+                        + 4 * (z >= split[2]).int())
+                sets = [torch.nonzero(code == o).flatten() for o in range(8)]
+            else:
+                sets = [None]
+            masks = np.zeros((len(loops), 8), np.uint32)
+            base = S.rule_loopons_per_loop(loops, ring=8)
+            two = band_loopons(loops, band) if band is not None else None
+            for o, idx in enumerate(sets):
+                if idx is not None and idx.numel() == 0:
+                    continue
+                sx, sy, sz = (t if idx is None else t[idx] for t in (x, y, z))
+                olo = [float(t.min()) for t in (sx, sy, sz)]
+                ohi = [float(t.max()) for t in (sx, sy, sz)]
+                m, far = far_images(loops, box, olo, ohi, hn, float(near_tol), kx, info=info,
                                     order=order, k_band=k_band, band=band)
-            if far:
-                ctx.check(B.lib().disco_hip_plan_set_image_skip(
-                    self.near._h, masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
-                    len(loops)))
-                base = S.rule_loopons_per_loop(loops, ring=8)
-                two = band_loopons(loops, band) if band is not None else None
+                if not far:
+                    continue
+                if idx is None:
+                    masks[:] = m[:, None]
+                else:
+                    masks[:, o] = m
                 copies = []
                 for l, off, two_ring in far:
                     for part in (two[l] if two_ring else base[l]):
                         copies.append(B.LoopData(phys=part.phys, line=False, x=part.x + off[0],
                                                  y=part.y + off[1], z=part.z + off[2]))
-                parts.setdefault(hn, []).append((copies, 0, -1))
-                self.skipped = len(far)
-                self.skipped_band = sum(1 for f in far if f[2])
+                near_grids.append((idx, [(copies, 0, -1)]))
+                self.skipped += len(far)
+                self.skipped_band += sum(1 for f in far if f[2])
+            if near_grids:
+                if octants:
+                    self.near.set_image_skip(masks, split)
+                else:
+                    self.near.set_image_skip(masks[:, 0])
         if self.img > 1:
             base = S.rule_loopons_per_loop(loops, ring=8) if base is None else base
             parts.setdefault(hf, []).append(([p for ps in base for p in ps], self.img, 1))
+        for idx, pl in near_grids:
+            self.grids.append(_Grid(ctx, x, y, z, box, hn, pl, grid_groups, order, idx=idx))
         for h, pl in parts.items():
             self.grids.append(_Grid(ctx, x, y, z, box, h, pl, grid_groups, order))
 

@@ -208,6 +208,15 @@ int disco_hip_line_band(const disco_line_loop *loops, int nloops, double *out);
  * say as a cubature on a grid when it lies far from every atom).  skip = NULL
  * clears every mask.  Synchronous. */
 int disco_hip_plan_set_image_skip(disco_plan *plan, const unsigned *skip, int nloops);
+/* (No counterpart in the reference.)  The same per octant of the atoms: an
+ * atom at (x, y, z) is in octant o = [x >= split[0]] + 2 [y >= split[1]] +
+ * 4 [z >= split[2]], and skip[8 l + o] is the mask of input loop l for the
+ * atoms of octant o (a grid per octant can then take the images far from that
+ * octant's atoms only).  An image left out for an atom must lie beyond the near
+ * slab of that atom (as every image far enough for a cubature does).  skip =
+ * NULL clears every mask; split may not hold NaN.  Synchronous. */
+int disco_hip_plan_set_image_skip_octants(disco_plan *plan, const unsigned *skip, int nloops,
+                                          const double *split);
 /* Host only: for each loop, {eligible (0/1), centre weight, ring weight, ring
  * radius, cut radius} of its 9-point cubature (out: 5 doubles per loop). */
 int disco_hip_line_cubature(const disco_line_loop *loops, int nloops, double *out);

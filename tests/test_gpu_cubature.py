@@ -69,12 +69,13 @@ def _near_cut_atoms(loops, rng, per_loop, dist):
     return np.concatenate(pts)
 
 
-@pytest.mark.parametrize("k_band", [None, 3.0])
-def test_c5_images_4(hip_ctx, k_band):
+@pytest.mark.parametrize("k_band,octants", [(None, True), (3.0, True), (3.0, False)])
+def test_c5_images_4(hip_ctx, k_band, octants):
     """200 loops drawn as the 4M-atom sample's, images +-4: 4,096 random atoms of
     the box and 4,000 atoms within 6 A of the loop cuts (the near slab, where the
     sum is exact in FP64) against the FP64 plain sum over all 729 images; the
-    cubature form and its two-ring band from 3 R."""
+    cubature form and its two-ring band from 3 R, with the +-1 images on a grid
+    per octant of the atoms or on one grid."""
     s = G.config_c2(seed=20245, nloops=200)
     rng = np.random.default_rng(4)
     L = s.box[0]
@@ -84,7 +85,7 @@ def test_c5_images_4(hip_ctx, k_band):
     x, y, z = _dev(P[:, 0], P[:, 1], P[:, 2])
     ref = _plain64(hip_ctx, x, y, z, s.loops, s.box, 4)
     cs = C.CubatureSum(hip_ctx, x, y, z, s.loops, s.box, 4, K_EXACT, near_tol=NEAR_TOL,
-                       k_band=k_band)
+                       k_band=k_band, octants=octants)
     import torch
     u = [torch.zeros_like(x) for _ in range(3)]
     cs.run(x, y, z, *u)
@@ -93,7 +94,8 @@ def test_c5_images_4(hip_ctx, k_band):
     assert cs.band == (0 if k_band is None else sum(len(lp.vx) >= 12 for lp in s.loops))
     cs.free()
     e = _err(u, ref)
-    print(f"C5 cubature (band {k_band}) vs FP64 plain (+-4), {x.numel()} atoms: {e:.3e} A")
+    print(f"C5 cubature (band {k_band}, octants {octants}) vs FP64 plain (+-4), {x.numel()} "
+          f"atoms: {e:.3e} A")
     assert e < TOL
 
 
@@ -133,6 +135,49 @@ def test_image_skip_leaves_exactly_those_images_out(hip_ctx):
     assert all(torch.equal(a, b) for a, b in zip(v, ref))
     p.free()
     q.free()
+
+
+def test_image_skip_per_octant(hip_ctx):
+    """FP64: a plan whose atoms leave images out by octant (a random mask per
+    loop and octant, planes through the middle of the atoms), plus, for the atoms
+    of each octant, the images that octant left out (a plan skipping the
+    complement), equals the plain sum; every atom takes its own octant's mask."""
+    import torch
+    rng = np.random.default_rng(11)
+    loops = kind_loops("glide", rng, 2) + kind_loops("mixed", rng, 1) + kind_loops("bent", rng, 1)
+    x, y, z = _dev(*atoms_near(loops, rng, nrand=900, per_loop=60))
+    ref = _plain64(hip_ctx, x, y, z, loops, BOX, 1)
+    split = [float(t.median()) for t in (x, y, z)]
+    masks = rng.integers(0, 1 << 27, size=(len(loops), 8), dtype=np.int64).astype(np.uint32)
+    p = B.Plan(hip_ctx, loops, BOX, 1, B.PREC_FP64)
+    p.set_image_skip(masks, split)
+    u = [torch.zeros_like(x) for _ in range(3)]
+    p.run_torch(x, y, z, *u)
+    code = (x >= split[0]).int() + 2 * (y >= split[1]).int() + 4 * (z >= split[2]).int()
+    full = (1 << 27) - 1
+    for o in range(8):
+        idx = torch.nonzero(code == o).flatten()
+        assert idx.numel() > 0
+        q = B.Plan(hip_ctx, loops, BOX, 1, B.PREC_FP64)
+        q.set_image_skip(np.uint32(full) ^ masks[:, o])
+        xo, yo, zo = (t[idx].contiguous() for t in (x, y, z))
+        v = [torch.zeros_like(xo) for _ in range(3)]
+        q.run_torch(xo, yo, zo, *v)
+        for a, b in zip(u, v):
+            a.index_add_(0, idx, b)
+        q.free()
+    torch.cuda.synchronize()
+    e = _err(u, ref)
+    assert e < 1e-10, e
+    # clearing restores the plain sum bit for bit; a NaN plane is refused
+    p.set_image_skip(None)
+    w = [torch.zeros_like(x) for _ in range(3)]
+    p.run_torch(x, y, z, *w)
+    torch.cuda.synchronize()
+    assert all(torch.equal(a, b) for a, b in zip(w, ref))
+    with pytest.raises(RuntimeError):
+        p.set_image_skip(masks, [float("nan"), 0.0, 0.0])
+    p.free()
 
 
 @pytest.mark.parametrize("k_rule,k_band", [(None, None), (8.0, None), (None, 3.0), (8.0, 3.0)])
