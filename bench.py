@@ -41,9 +41,10 @@ FAR_PASS_HBM_BYTES_PER_ATOM = 48.50
 # cubature form of the sample (reported beside the plain sum): exact within 3 cut
 # radii of a loop image, the two-ring cubature from 3 to 5 radii (loops it
 # represents; the others stay exact there), the 9-point cubature beyond; +-1
-# images far from every atom on an L/32 grid (interpolation error bound 1e-7 A
-# each), the far shell on an L/12 grid, both interpolated by 6-node stencils
-K_EXACT, K_BAND, NEAR_TOL = 5.0, 3.0, 1e-7
+# images on an L/32 grid per octant of the atoms (those far from the octant's
+# atoms; interpolation error bound 3e-7 A each), the far shell on an L/12 grid
+# (shells 3 and beyond on L/6), all interpolated by 6-node stencils
+K_EXACT, K_BAND, NEAR_TOL = 5.0, 3.0, 3e-7
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on this workload: "
                   "profiles/line_kernel_far__fcc4m_50loops_img2__pmc_traffic.md")
 
@@ -285,8 +286,9 @@ def main():
                "form": "exact line integral within k_band R of each loop image (FP32, FP64 "
                        "near slab; to k_exact R for loops the two-ring cubature does not "
                        "represent), the two-ring cubature (25 points) to k_exact R, the 9-point "
-                       "cubature of the cut beyond; +-1 images far from every atom on an L/32 "
-                       "grid, the far shell on an L/12 grid (6-node stencils)"}
+                       "cubature of the cut beyond; +-1 images far from the atoms of an octant "
+                       "on that octant's L/32 grid, the far shell on an L/12 grid (6-node "
+                       "stencils)"}
         cs.free()
         del w
         print(f"[bench] cubature form: {cms:.1f} ms per sample ({ms_step / cms:.1f}x the plain "

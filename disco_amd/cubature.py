@@ -184,10 +184,12 @@ class CubatureSum:
     L/32) interpolates their field to within near_tol (far_images) are taken
     out of the line plan and summed as cubatures at that grid's nodes (None:
     every +-1 image stays in the line plan).  The far shell (1, img] is summed
-    as cubatures on a grid of spacing grid_h (default L/12).  Both grids are
-    interpolated by 6-node (quintic) stencils (order = 4: cubic, with defaults
-    L/64 and L/24).  Defaults: measured on the 4M-atom samples at images +-2
-    and +-4 (max |du| 1.0e-6 and 2.1e-6 A against the FP64 plain sum, every atom).
+    as cubatures on a grid of spacing grid_h (default L/12), its images beyond
+    shell 2 on one of spacing outer_h (default L/6; None: on the grid_h grid
+    too).  The grids are interpolated by 6-node (quintic) stencils (order = 4:
+    cubic, with defaults L/64 and L/24 and no outer grid).  Defaults: measured
+    on the 4M-atom samples at images +-2 and +-4 (max |du| 1.0e-6 and 2.1e-6 A
+    against the FP64 plain sum, every atom).
 
     k_band (None: off): lanes between k_band and k_exact cut radii of a loop
     image take the two-ring cubature (centre and two rings of twelve, error
@@ -205,8 +207,8 @@ class CubatureSum:
     from every atom."""
 
     def __init__(self, ctx, x, y, z, loops, box, img, k_exact=5.0, k_rule=None,
-                 grid_h=None, grid_groups=None, near_tol=1e-7, near_h=None, order=6,
-                 k_band=None, octants=True):
+                 grid_h=None, grid_groups=None, near_tol=3e-7, near_h=None, order=6,
+                 k_band=None, octants=True, outer_h=-1.0):
         self.ctx = ctx
         self.img = int(img)
         self.near = B.Plan(ctx, loops, box, min(1, self.img), B.PREC_FP32C)
@@ -275,7 +277,16 @@ class CubatureSum:
                     self.near.set_image_skip(masks[:, 0])
         if self.img > 1:
             base = S.rule_loopons_per_loop(loops, ring=8) if base is None else base
-            parts.setdefault(hf, []).append(([p for ps in base for p in ps], self.img, 1))
+            flat = [p for ps in base for p in ps]
+            if outer_h is not None and outer_h < 0:
+                outer_h = min(box) / 6.0 if order == 6 else None
+            if self.img > 2 and outer_h:
+                # shell 2 on the grid_h grid, shells 3..img on the coarser
+                # outer_h grid (their field is smoother at the atoms)
+                parts.setdefault(hf, []).append((flat, 2, 1))
+                parts.setdefault(float(outer_h), []).append((flat, self.img, 2))
+            else:
+                parts.setdefault(hf, []).append((flat, self.img, 1))
         for idx, pl in near_grids:
             self.grids.append(_Grid(ctx, x, y, z, box, hn, pl, grid_groups, order, idx=idx))
         for h, pl in parts.items():
@@ -298,7 +309,7 @@ class CubatureSum:
 
 
 def cubature_sum_torch(ctx, x, y, z, loops, box, img, k_exact=5.0, k_rule=None, u=None,
-                       accumulate=False, grid_h=None, near_tol=1e-7, near_h=None, k_band=None):
+                       accumulate=False, grid_h=None, near_tol=3e-7, near_h=None, k_band=None):
     """One-shot CubatureSum: returns u (three device tensors)."""
     import torch
     if u is None:
