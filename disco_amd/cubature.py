@@ -5,9 +5,9 @@ evaluates the closed-form line integral (Code/barnett.hpp:44-107) of every
 loop at every image.  Far from a loop image its field is that of the Kroupa
 term (Code/kroupa.hpp:52-73) integrated over the cut, which a 9-point cubature
 (the centre and a ring of eight matching the cut's area and its second and
-fourth radial moments) reproduces to O((R/r)^8) of |b|: measured on the CPU
-oracle, at most about 0.15 (R/r)^8 A for |b| = 2.56 A (circles and regular
-hexagons).  The cubature form is
+fourth radial moments) reproduces to O((R/r)^8) of |b|: measured against the
+FP64 line integral on the CPU, at most about 0.15 (R/r)^8 A for |b| = 2.56 A
+(circles and regular hexagons).  The cubature form is
 
 * the +-1 shell by an FP32C line plan whose far pass takes, per lane and loop
   image, the exact field within k_exact cut radii of the image's centre, the
@@ -19,8 +19,10 @@ hexagons).  The cubature form is
   length from every atom, by the same cubature summed by the loopon kernels at
   the nodes of a grid of spacing ``grid_h`` over the atoms and interpolated to
   them by 6-node Lagrange (``disco_hip_grid_interp``); so are the +-1 images so
-  far from every atom that the cubature alone is accurate at all of them and
-  a finer grid interpolates their field to a stated tolerance (``far_images``).
+  far from the atoms of an octant of the atom box that the cubature alone is
+  accurate at all of them and a finer grid over that octant interpolates
+  their field to a stated tolerance (``far_images``; the line plan leaves them
+  out for those atoms only, ``Plan.set_image_skip`` with the octant planes).
   A loop whose cut is not a flat region about its apex (off its plane, lobes
   of opposite sense, ...) is summed there as Radon's rule on each of its fan
   triangles instead (``surrogate.fan_loopons``), and its +-1 images stay in
