@@ -550,7 +550,8 @@ __device__ __forceinline__ void atom_loopon(double px, double py, double pz,
 // 4M-atom sample, the same bits); the global-axes FP64 walk keeps 3 (168
 // registers, no scratch); the FP32 ones use at most 74 registers either way.
 template <int PASS, int KIND, bool CUBATURE = false, bool SKIPS = false>
-__global__ __launch_bounds__(kBlock, (PASS == PASS_FP64 && KIND == FP64_FRAME) ? 5 : 3) void line_kernel(
+__global__
+__launch_bounds__(kBlock, (PASS == PASS_FP64 && KIND == FP64_FRAME) ? 5 : 3) void line_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
     const LineLoop* __restrict__ loops, int nloops, const SegF* __restrict__ segf,
@@ -560,8 +561,8 @@ __global__ __launch_bounds__(kBlock, (PASS == PASS_FP64 && KIND == FP64_FRAME) ?
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    atom_line<PASS, KIND, CUBATURE, SKIPS>(x[i], y[i], z[i], loops, 0, nloops, segf, segd, Lx, Ly, Lz,
-                                    img, inner, sx, sy, sz);
+    atom_line<PASS, KIND, CUBATURE, SKIPS>(x[i], y[i], z[i], loops, 0, nloops, segf, segd, Lx,
+                                           Ly, Lz, img, inner, sx, sy, sz);
     if (accumulate) { sx += ux[i]; sy += uy[i]; sz += uz[i]; }
     ux[i] = sx; uy[i] = sy; uz[i] = sz;
 }
