@@ -452,6 +452,8 @@ class Plan:
         m = np.ascontiguousarray(masks, dtype=np.uint32)
         ptr = m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint))
         if split is None:
+            if m.ndim != 1:
+                raise ValueError("masks per octant need the octant planes (split)")
             self.ctx.check(lib().disco_hip_plan_set_image_skip(self._h, ptr, m.shape[0]))
             return
         if m.ndim != 2 or m.shape[1] != 8:

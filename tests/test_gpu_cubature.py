@@ -177,6 +177,10 @@ def test_image_skip_per_octant(hip_ctx):
     assert all(torch.equal(a, b) for a, b in zip(w, ref))
     with pytest.raises(RuntimeError):
         p.set_image_skip(masks, [float("nan"), 0.0, 0.0])
+    with pytest.raises(ValueError):
+        p.set_image_skip(masks)                     # octant masks without their planes
+    with pytest.raises(ValueError):
+        p.set_image_skip(masks[:, :4], split)       # not eight per loop
     p.free()
 
 
