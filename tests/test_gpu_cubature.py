@@ -184,6 +184,58 @@ def test_image_skip_per_octant(hip_ctx):
     p.free()
 
 
+def test_image_skip_per_octant_fp32c_every_loop_form(hip_ctx):
+    """FP32C far and near passes without the cubature, loops of every form
+    (global axes, flat, glide, prismatic): images far from an octant's atoms left
+    out for those atoms, plus those images summed for the octant's atoms by plans
+    that keep only them, equals the plain FP32C sum (each image's FP32 field is
+    the same in both; only the FP64 sums' order differs)."""
+    import torch
+    rng = np.random.default_rng(12)
+    loops = (kind_loops("glide", rng, 2) + kind_loops("mixed", rng, 1) + kind_loops("prism", rng, 1)
+             + kind_loops("bent", rng, 1))
+    x, y, z = _dev(*atoms_near(loops, rng, nrand=900, per_loop=60))
+    ref = [torch.zeros_like(x) for _ in range(3)]
+    p0 = B.Plan(hip_ctx, loops, BOX, 1, B.PREC_FP32C)
+    p0.run_torch(x, y, z, *ref)
+    split = [float(t.median()) for t in (x, y, z)]
+    code = (x >= split[0]).int() + 2 * (y >= split[1]).int() + 4 * (z >= split[2]).int()
+    P = np.stack([t.cpu().numpy() for t in (x, y, z)], 1)
+    oc = code.cpu().numpy()
+    masks = np.zeros((len(loops), 8), np.uint32)
+    for l, lp in enumerate(loops):
+        c = np.array([lp.cx, lp.cy, lp.cz])
+        R = np.linalg.norm(np.stack([lp.vx, lp.vy, lp.vz], 1) - c, axis=1).max()
+        for o in range(8):
+            lo, hi = P[oc == o].min(0), P[oc == o].max(0)
+            for b in range(27):
+                ci = c + np.array([b // 9 - 1, (b // 3) % 3 - 1, b % 3 - 1]) * np.array(BOX)
+                gap = np.maximum(np.maximum(lo - ci, ci - hi), 0.0)
+                if np.linalg.norm(gap) > R + 40.0:        # beyond every near slab
+                    masks[l, o] |= np.uint32(1 << b)
+    assert masks.any() and (masks != (1 << 27) - 1).any()    # some images left out, not all
+    p = B.Plan(hip_ctx, loops, BOX, 1, B.PREC_FP32C)
+    p.set_image_skip(masks, split)
+    u = [torch.zeros_like(x) for _ in range(3)]
+    p.run_torch(x, y, z, *u)
+    full = (1 << 27) - 1
+    for o in range(8):
+        idx = torch.nonzero(code == o).flatten()
+        q = B.Plan(hip_ctx, loops, BOX, 1, B.PREC_FP32C)
+        q.set_image_skip(np.uint32(full) ^ masks[:, o])
+        xo, yo, zo = (t[idx].contiguous() for t in (x, y, z))
+        v = [torch.zeros_like(xo) for _ in range(3)]
+        q.run_torch(xo, yo, zo, *v)
+        for a, b in zip(u, v):
+            a.index_add_(0, idx, b)
+        q.free()
+    torch.cuda.synchronize()
+    e = _err(u, ref)
+    assert e < 1e-11, e
+    p.free()
+    p0.free()
+
+
 @pytest.mark.parametrize("k_rule,k_band", [(None, None), (8.0, None), (None, 3.0), (8.0, 3.0)])
 def test_loop_forms_and_window(hip_ctx, k_rule, k_band):
     """Glide, prismatic and general-b circles (the rule represents them), loops
