@@ -586,8 +586,11 @@ __global__ __launch_bounds__(kBlock, 3) void line_near_kernel(
     }
 }
 
+// The FP32 loopon walk fits 64 registers: 8 waves per SIMD (81 registers and 6
+// waves without the bound; 200.7 -> 182.4 ms on the 432,000-atom loopon sample,
+// the same bits).
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void loopon_kernel(
+__global__ __launch_bounds__(kBlock, PREC == DISCO_PREC_FP32C ? 8 : 1) void loopon_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     double* __restrict__ ux, double* __restrict__ uy, double* __restrict__ uz, int n,
     const LooponLoop* __restrict__ loops, int nloops, const float4* __restrict__ pf,
@@ -608,7 +611,7 @@ __global__ __launch_bounds__(kBlock) void loopon_kernel(
 // has split times as many workgroups; each group's sum goes to its own slice of
 // `partial`, and loopon_split_sum_kernel adds the slices in group order.
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void loopon_split_kernel(
+__global__ __launch_bounds__(kBlock, PREC == DISCO_PREC_FP32C ? 8 : 1) void loopon_split_kernel(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     int n, const LooponLoop* __restrict__ loops, int nloops, const float4* __restrict__ pf,
     const double4* __restrict__ pd, double Lx, double Ly, double Lz, int img, int inner,
